@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X ELP ring engine on one GPU.
+
+A step is one time unit over the whole box: one launch of the move kernel
+(every tile of the shifted tile grid: stage in LDS, n_iter iterations of
+{TransForw, DiffuseSL, TransForw, TransBack}, write back).  The workload is
+the dense ring melt the target is quoted on: L=128, N=1000, density 1.2
+(2517 rings), built by the drop-in's own start-up (compact cube rings +
+stored-length reshuffle) from seed 0 and relaxed by the warm-up steps.
+
+Prints one JSON line.  value = move attempts/s (every slot of every substep
+counts as one attempt, as in the reference kernel).  roofline.achieved =
+algorithmic HBM bytes per launch (the tile is read and written once:
+2 B/site x L^3) / average launch time from HIP events on the engine's stream.
+cpu_baseline = the efpol restatement (oracle/bin/efpol_port_ring) timed on
+the host cores, one independent process per core, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mc-polymer-simulation_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(L, N, density, cores, warm, timed):
+    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
+    if not os.path.exists(exe):
+        return None
+    procs = [subprocess.Popen([exe, "bench", str(L), str(N), str(density), str(1000 + i), str(timed), str(warm)],
+                              stdout=subprocess.PIPE, text=True) for i in range(cores)]
+    rate = 0.0
+    for p in procs:
+        out, _ = p.communicate()
+        kv = dict(x.split("=") for x in out.split())
+        rate += float(kv["attempts"]) / float(kv["seconds"])
+    return {
+        "value": rate,
+        "unit": "attempts/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"efpol restatement, ring L={L} N={N} density {density}, {cores} independent processes "
+                   f"(seeds 1000..{1000 + cores - 1}), {warm} untimed + {timed} timed time units from efpol's "
+                   f"two-site-loop start; DoMCStep region only; 1 attempt = 1 monomer move attempt"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--L", type=int, default=128)
+    ap.add_argument("--N", type=int, default=1000)
+    ap.add_argument("--density", type=float, default=1.2)
+    ap.add_argument("--tile", default="16")
+    ap.add_argument("--sublattice", default="dense", choices=["cubic", "dense"])
+    ap.add_argument("--n-iter", type=int, default=None)
+    ap.add_argument("--cpu-cores", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent replicas shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import elp
+
+    tile = tuple(int(x) for x in args.tile.split("x")) if "x" in args.tile else (int(args.tile),) * 3
+    dense = args.sublattice == "dense"
+    n_iter = args.n_iter or (13 if dense else 51)
+    host = elp.RingHost(N=args.N, L=args.L, density=args.density, seed=0)
+    eng = elp.Engine("ring", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC, n_iter)
+    eng.upload(host.lattice())
+    host.load_shift(tile)
+    nsite = host.L[0] * host.L[1] * host.L[2]
+    attempts_per_step = n_iter * 4 * (nsite // 64) * (4 if dense else 1)
+
+    tau = 0
+    if args.warmup:
+        org = host.origins(tile, args.warmup)
+        eng.run(org, tau0=tau, seed=host.seed)
+        tau += args.warmup
+    org = host.origins(tile, args.steps)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.run(org, tau0=tau, seed=host.seed, sync=False)
+    eng.sync()
+    t1 = time.perf_counter()
+    kernel_ms = eng.last_run_ms()
+    wall = t1 - t0
+    # the lattice must still hold every labelled ring
+    host.set_lattice(eng.download())
+
+    value = attempts_per_step * args.steps / wall
+    launch_s = kernel_ms * 1e-3 / args.steps
+    bytes_per_launch = 2 * nsite
+    achieved = bytes_per_launch / launch_s / 1e9
+    info = eng.launch_info()
+
+    cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+    cpu = None if args.no_cpu else cpu_baseline(args.L, args.N, args.density, cores, warm=300, timed=100)
+
+    rec = {
+        "metric": "MC move attempts/s, %HBM roofline; equil. wall-time learned vs plain",
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ring melt L={host.L[0]} N={host.N} density {args.density} ({host.n_pol} rings), "
+                        f"one time unit per step",
+            "tile": list(tile),
+            "sublattice": args.sublattice,
+            "iterations_per_time_unit": n_iter,
+            "attempts_per_site_per_time_unit": attempts_per_step / nsite,
+            "threads_per_block": info["threads"],
+            "blocks_per_launch": info["blocks"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "ring_time_unit",
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": launch_s * 1e6,
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        rec["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(rec))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,44 @@
+// elp_device.h -- device helpers shared by the gfx950 move kernels:
+// Philox4x32-10 counter-based generator and FCC bond-code algebra.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace elp {
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC11).  Round: two 32x32->64 products of counter words
+// 0 and 2 with the multipliers; key bumped by the Weyl constants between
+// rounds.  Ten rounds.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint32_t word_of(const uint4& w, int j) {
+    return j == 0 ? w.x : j == 1 ? w.y : j == 2 ? w.z : w.w;
+}
+
+// Counter word 2 of the tile-uniform site-offset stream; slot streams use
+// the slot index (always < 2^31) there instead.
+constexpr uint32_t kOffsetStream = 0xFFFFFFFFu;
+
+// 4-bit bond code (t,u,v,w) -> displacement (t-w, u-w, v-w)
+__device__ __forceinline__ int bdt(uint32_t b) { return (int)(b & 1u) - (int)(b >> 3); }
+__device__ __forceinline__ int bdu(uint32_t b) { return (int)((b >> 1) & 1u) - (int)(b >> 3); }
+__device__ __forceinline__ int bdv(uint32_t b) { return (int)((b >> 2) & 1u) - (int)(b >> 3); }
+__device__ __forceinline__ uint32_t bond_add(uint32_t a, uint32_t b) { return ((a | b) == 0xfu) ? (a & b) : (a | b); }
+// a+b is a single bond (codes 0x3 and 0xc are not bonds)
+__device__ __forceinline__ bool bond_add_valid(uint32_t a, uint32_t b, uint32_t& c) {
+    if ((a | b) != 0xfu && (a & b)) return false;
+    c = bond_add(a, b);
+    return c != 0x3u && c != 0xcu;
+}
+
+}  // namespace elp

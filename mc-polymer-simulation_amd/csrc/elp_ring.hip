@@ -1,0 +1,260 @@
+// elp_ring.hip -- gfx950 move kernel of the ring-polymer ELP engine.
+//
+// One workgroup owns one tile of the box for one time unit: it stages the
+// tile (1 byte per site) from HBM into LDS, runs n_iter iterations of the
+// substeps {TransForw, DiffuseSL, TransForw, TransBack} (the ringpol3 move
+// set, gpupol2/ringpol3.cu:92-218 and :278-292) and writes the tile back in
+// place.  Tiles of one time unit partition the box, so the in-place update
+// is race-free; the grid origin moves between time units (host shift).
+//
+// Within a substep every slot of the tile attempts the same move type at a
+// site of one coset of an independent sub-lattice, chosen per tile and
+// substep from the tile-uniform offset stream:
+//   cubic : one site per 4^3 cell, coset offset 6 bits (t,u,v mod 4)
+//           -- the reference's workgroup-uniform cell offset;
+//   dense : four sites per 4^3 cell, the coset of 4Z^3 + <(1,3,2)> with
+//           representative (0, c&3, c>>2); slot j of a cell is the point
+//           rep + j*(1,3,2) mod 4.
+// Any two sites of one coset differ by a lattice vector of squared length
+// >= 10 (FCC units, bond = 2), so the sites a move reads or writes (the site
+// and its 12 neighbours) never overlap between slots: all slots of a
+// substep commute, and a sequential replay in any order is exact.
+// Moves that would touch a site outside the tile are rejected, as in the
+// reference (OOB, gpupol2/ringpol3.cu:71-73).
+//
+// Randomness (Philox4x32-10, key = (seed, stream)):
+//   offsets : counter (tau, tile, 0xFFFFFFFF, iter/4), word iter%4,
+//             byte s = substep s;
+//   moves   : counter (tau, tile, slot, iter/4), word iter%4, byte s.
+// LDS layout: cell-major, addr = cell + NCELL * (t&3 | (u&3)<<2 | (v&3)<<4),
+// so the site reads of one substep hit consecutive bytes.
+#include "elp_device.h"
+#include "elp_kernels.h"
+
+namespace elp {
+
+template <int TT, int TU, int TV>
+struct RingTile {
+    static constexpr int NCT = TT / 4, NCU = TU / 4, NCV = TV / 4;
+    static constexpr int NCELL = NCT * NCU * NCV;
+    static constexpr int NBYTE = TT * TU * TV;
+    __device__ static __forceinline__ int addr(int t, int u, int v) {
+        const int cell = (t >> 2) + NCT * ((u >> 2) + NCU * (v >> 2));
+        return cell + NCELL * ((t & 3) | ((u & 3) << 2) | ((v & 3) << 4));
+    }
+    __device__ static __forceinline__ bool inside(int t, int u, int v) {
+        return (unsigned)t < (unsigned)TT && (unsigned)u < (unsigned)TU && (unsigned)v < (unsigned)TV;
+    }
+};
+
+// Insert a monomer on an empty common neighbour of the site and its bond
+// partner, taking it from whichever end holds stored length
+// (gpupol2/ringpol3.cu:92-145).  r: bit0 decomposition, bit1 order,
+// bit2 which monomer moves when both ends hold stored length.
+template <class G>
+__device__ __forceinline__ void trans_forw(uint8_t* lat, int t, int u, int v, uint32_t r, const uint32_t* trans) {
+    const int a = G::addr(t, u, v);
+    uint32_t s = lat[a];
+    if (!s) return;
+    const uint32_t next = s & 0xfu, label = s & 0x30u, sl = s & 0x40u;
+    const int nt = t + bdt(next), nu = u + bdu(next), nv = v + bdv(next);
+    if (!G::inside(nt, nu, nv)) return;
+    const int na = G::addr(nt, nu, nv);
+    uint32_t ns = lat[na];
+    const uint32_t nsl = ns & 0x40u;
+    if (!(sl | nsl)) return;
+    uint32_t nb1 = (trans[next >> 2] >> (4 * (2 * (next & 3u) + (r & 1u)))) & 0xfu;
+    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
+    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+    const int dt = t + bdt(nb1), du = u + bdu(nb1), dv = v + bdv(nb1);
+    if (!G::inside(dt, du, dv)) return;
+    const int da = G::addr(dt, du, dv);
+    if (lat[da]) return;
+    const bool moveFirst = (sl && nsl) ? ((r >> 2) & 1u) : (sl != 0);
+    uint32_t d = nb2;
+    if (moveFirst) {
+        s = nb1 | ((label >> 1) & 0x10u);
+        d |= label & 0x10u;
+    } else {
+        s = nb1 | label | sl;
+        d |= (ns & 0x20u) >> 1;
+        ns &= 0x1fu;
+    }
+    lat[a] = (uint8_t)s;
+    lat[da] = (uint8_t)d;
+    if (!moveFirst) lat[na] = (uint8_t)ns;
+}
+
+// Retract a corner: site -> src -> third site forming a triangle; the src
+// monomer folds onto one of its neighbours as stored length
+// (gpupol2/ringpol3.cu:147-192).  r: bit0 choice when neither end has SL.
+template <class G>
+__device__ __forceinline__ void trans_back(uint8_t* lat, int t, int u, int v, uint32_t r) {
+    const int a = G::addr(t, u, v);
+    uint32_t s = lat[a];
+    if (!s) return;
+    const uint32_t next = s & 0xfu, label = s & 0x30u, sl = s & 0x40u;
+    const int st = t + bdt(next), su = u + bdu(next), sv = v + bdv(next);
+    if (!G::inside(st, su, sv)) return;
+    const int sa = G::addr(st, su, sv);
+    const uint32_t ss = lat[sa];
+    if (ss & 0x40u) return;
+    uint32_t newNext;
+    if (!bond_add_valid(next, ss & 0xfu, newNext)) return;
+    const int wt = t + bdt(newNext), wu = u + bdu(newNext), wv = v + bdv(newNext);
+    if (!G::inside(wt, wu, wv)) return;
+    const int wa = G::addr(wt, wu, wv);
+    uint32_t ws = lat[wa];
+    const uint32_t wsl = ws & 0x40u;
+    if (sl && wsl) return;
+    const bool moveFirst = (!sl && !wsl) ? (r & 1u) : (sl == 0);
+    const uint32_t srcLabel = ss & 0x30u;
+    if (moveFirst) {
+        s = newNext | (label << 1) | srcLabel | 0x40u;
+    } else {
+        s = newNext | label | sl;
+        ws = (ws & 0x3fu) | (srcLabel << 1) | 0x40u;
+    }
+    lat[sa] = 0;
+    lat[a] = (uint8_t)s;
+    lat[wa] = (uint8_t)ws;
+}
+
+// Move one stored-length unit across the bond when exactly one end holds it
+// (gpupol2/ringpol3.cu:194-218).
+template <class G>
+__device__ __forceinline__ void diffuse_sl(uint8_t* lat, int t, int u, int v) {
+    const int a = G::addr(t, u, v);
+    uint32_t s = lat[a];
+    if (!s) return;
+    const uint32_t next = s & 0xfu, label = s & 0x30u, sl = s & 0x40u;
+    const int nt = t + bdt(next), nu = u + bdu(next), nv = v + bdv(next);
+    if (!G::inside(nt, nu, nv)) return;
+    const int na = G::addr(nt, nu, nv);
+    uint32_t ns = lat[na];
+    if (((ns & 0x40u) + sl) != 0x40u) return;
+    if (sl) {
+        ns = ns | ((label & 0x10u) << 1) | 0x40u;
+        s = next | ((label >> 1) & 0x10u);
+    } else {
+        s = next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u;
+        ns &= 0x1fu;
+    }
+    lat[a] = (uint8_t)s;
+    lat[na] = (uint8_t)ns;
+}
+
+template <int TT, int TU, int TV, bool DENSE, int THREADS>
+__global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ glat, RingLaunch p) {
+    using G = RingTile<TT, TU, TV>;
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NSLOT = G::NCELL * SPC;
+    constexpr int SPT = NSLOT / THREADS;
+    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+    __shared__ uint8_t tile[G::NBYTE];
+
+    const int tid = threadIdx.x;
+    const uint32_t tileId = blockIdx.x;
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+
+    // HBM -> LDS (periodic wrap of the shifted tile)
+    for (int i = tid; i < G::NBYTE; i += THREADS) {
+        const int lt = i % TT, lu = (i / TT) % TU, lv = i / (TT * TU);
+        int gt = gt0 + lt, gu = gu0 + lu, gv = gv0 + lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        tile[G::addr(lt, lu, lv)] = glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+
+    // per-slot cell origin
+    int ct[SPT], cu[SPT], cv[SPT], jj[SPT];
+    uint32_t slot[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        slot[k] = (uint32_t)(tid + k * THREADS);
+        const int cell = (int)(slot[k] / SPC);
+        jj[k] = (int)(slot[k] % SPC);
+        ct[k] = 4 * (cell % G::NCT);
+        cu[k] = 4 * ((cell / G::NCT) % G::NCU);
+        cv[k] = 4 * (cell / (G::NCT * G::NCU));
+    }
+    __syncthreads();
+
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        uint4 sw[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+        for (int j = 0; j < nj; j++) {
+            const uint32_t offw = word_of(ow, j);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+#pragma unroll
+                for (int k = 0; k < SPT; k++) {
+                    int t, u, v;
+                    if (DENSE) {
+                        t = ct[k] + jj[k];
+                        u = cu[k] + (int)((c + 3u * (uint32_t)jj[k]) & 3u);
+                        v = cv[k] + (int)(((c >> 2) + 2u * (uint32_t)jj[k]) & 3u);
+                    } else {
+                        t = ct[k] + (int)(c & 3u);
+                        u = cu[k] + (int)((c >> 2) & 3u);
+                        v = cv[k] + (int)((c >> 4) & 3u);
+                    }
+                    const uint32_t r = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
+                    if (s == 1) diffuse_sl<G>(tile, t, u, v);
+                    else if (s == 3) trans_back<G>(tile, t, u, v, r);
+                    else trans_forw<G>(tile, t, u, v, r, p.trans);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    // LDS -> HBM
+    for (int i = tid; i < G::NBYTE; i += THREADS) {
+        const int lt = i % TT, lu = (i / TT) % TU, lv = i / (TT * TU);
+        int gt = gt0 + lt, gu = gu0 + lu, gv = gv0 + lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] = tile[G::addr(lt, lu, lv)];
+    }
+}
+
+// ---- dispatch over the compiled schedules ----
+#define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                   \
+    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                  \
+        if (probe) return 0;                                                                    \
+        hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, \
+                           stream, glat, p);                                                    \
+        return (int)hipGetLastError();                                                          \
+    }
+
+int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
+                uint8_t* glat, const RingLaunch& p) {
+    ELP_RING_CASE(16, 16, 16, false, 64)
+    ELP_RING_CASE(16, 16, 16, true, 64)
+    ELP_RING_CASE(16, 16, 16, true, 128)
+    ELP_RING_CASE(16, 16, 16, true, 256)
+    ELP_RING_CASE(32, 32, 32, false, 256)
+    ELP_RING_CASE(32, 32, 32, false, 512)
+    ELP_RING_CASE(32, 32, 32, true, 512)
+    ELP_RING_CASE(32, 32, 32, true, 1024)
+    ELP_RING_CASE(32, 32, 16, false, 256)
+    ELP_RING_CASE(32, 32, 16, true, 256)
+    ELP_RING_CASE(32, 32, 16, true, 512)
+    ELP_RING_CASE(16, 16, 32, false, 128)
+    ELP_RING_CASE(16, 16, 32, true, 128)
+    ELP_RING_CASE(16, 16, 32, true, 256)
+    return -1;
+}
+
+}  // namespace elp

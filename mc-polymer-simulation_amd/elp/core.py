@@ -1,0 +1,232 @@
+"""ctypes bindings of libelp.so and the drop-in host libraries."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+REPO_DIR = os.path.dirname(PKG_DIR)
+HEADER = os.path.join(REPO_DIR, "include", "elp_gpu.h")
+
+SUBLAT_CUBIC = 0
+SUBLAT_DENSE = 1
+
+
+class ElpError(RuntimeError):
+    pass
+
+
+_libs = {}
+
+
+def _load(name):
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise ElpError(f"{path} is not built: run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    _libs[name] = lib
+    return lib
+
+
+def load_elp():
+    lib = _load("libelp.so")
+    if not getattr(lib, "_elp_typed", False):
+        P, I, U32, U64 = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64
+        lib.elp_last_error.restype = C.c_char_p
+        lib.elp_device_count.argtypes = [C.POINTER(C.c_int)]
+        for fn in ("elp_ring_create", "elp_lin_create"):
+            getattr(lib, fn).argtypes = [C.POINTER(P), I, I, I, I, I, I, I, I, I]
+        lib.elp_destroy.argtypes = [P]
+        lib.elp_upload.argtypes = [P, P, U64]
+        lib.elp_download.argtypes = [P, P, U64]
+        lib.elp_run.argtypes = [P, I, P, U64, U32, U32]
+        lib.elp_sync.argtypes = [P]
+        lib.elp_last_run_ms.argtypes = [P, C.POINTER(C.c_float)]
+        lib.elp_launch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        lib._elp_typed = True
+    return lib
+
+
+def _check(rc, lib=None):
+    if rc != 0:
+        lib = lib or load_elp()
+        raise ElpError(f"elp error {rc}: {lib.elp_last_error().decode()}")
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(load_elp().elp_device_count(C.byref(n)))
+    return n.value
+
+
+def declared_symbols():
+    """extern "C" entry points declared in include/elp_gpu.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(elp_\w+)\s*\(", text, re.M)))
+
+
+def exported_symbols(path=None):
+    path = path or os.path.join(LIB_DIR, "libelp.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return sorted({ln.split()[-1] for ln in out.splitlines() if ln.strip()})
+
+
+class Engine:
+    """One device lattice + stream (ring: uint8 per site, linear: uint16)."""
+
+    def __init__(self, kind, L, tile=(32, 32, 32), sublattice=SUBLAT_CUBIC, n_iter=51, device=0):
+        self.lib = load_elp()
+        self.kind = kind
+        self.L = tuple(int(x) for x in (L if hasattr(L, "__len__") else (L, L, L)))
+        self.tile = tuple(int(x) for x in tile)
+        self.dtype = np.uint8 if kind == "ring" else np.uint16
+        h = C.c_void_p()
+        fn = self.lib.elp_ring_create if kind == "ring" else self.lib.elp_lin_create
+        _check(fn(C.byref(h), device, *self.L, *self.tile, int(sublattice), int(n_iter)), self.lib)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.elp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def shape_tuv(self):
+        return self.L
+
+    def upload(self, lat):
+        a = np.ascontiguousarray(lat, dtype=self.dtype)
+        _check(self.lib.elp_upload(self.h, a.ctypes.data, a.nbytes), self.lib)
+
+    def download(self):
+        a = np.empty(self.L[0] * self.L[1] * self.L[2], dtype=self.dtype)
+        _check(self.lib.elp_download(self.h, a.ctypes.data, a.nbytes), self.lib)
+        return a
+
+    def run(self, origins, tau0, seed, stream=0, sync=True):
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        _check(self.lib.elp_run(self.h, len(o), o.ctypes.data, int(tau0), int(seed) & 0xFFFFFFFF,
+                                int(stream) & 0xFFFFFFFF), self.lib)
+        if sync:
+            self.sync()
+
+    def sync(self):
+        _check(self.lib.elp_sync(self.h), self.lib)
+
+    def last_run_ms(self):
+        ms = C.c_float(0)
+        _check(self.lib.elp_last_run_ms(self.h, C.byref(ms)), self.lib)
+        return ms.value
+
+    def launch_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.elp_launch_info(self.h, C.byref(a), C.byref(b), C.byref(c)), self.lib)
+        return {"threads": a.value, "blocks": b.value, "lds_bytes": c.value}
+
+
+class _Host:
+    """Handle API of a drop-in host library (fresh start, shift stream, tuv)."""
+
+    prefix = None
+    libname = None
+    dtype = None
+
+    def __init__(self, N=None, L=None, density=1.2, seed=0, path=None):
+        self.lib = _load(self.libname)
+        p = self.prefix
+        lib = self.lib
+        if not getattr(lib, "_host_typed", False):
+            getattr(lib, p + "new").restype = C.c_void_p
+            getattr(lib, p + "new").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint32]
+            getattr(lib, p + "from_file").restype = C.c_void_p
+            getattr(lib, p + "from_file").argtypes = [C.c_char_p, C.c_uint32]
+            getattr(lib, p + "free").argtypes = [C.c_void_p]
+            getattr(lib, p + "info").argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 6
+            getattr(lib, p + "get_lattice").argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+            getattr(lib, p + "set_lattice").argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+            getattr(lib, p + "write_tuv").argtypes = [C.c_void_p, C.c_char_p]
+            getattr(lib, p + "polymer").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+            getattr(lib, p + "load_shift").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+            getattr(lib, p + "origins").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+            getattr(lib, p + "seed").restype = C.c_uint32
+            getattr(lib, p + "seed").argtypes = [C.c_void_p]
+            lib._host_typed = True
+        if path is not None:
+            h = getattr(lib, p + "from_file")(os.fsencode(path), int(seed))
+        else:
+            Ls = L if hasattr(L, "__len__") else (L, L, L)
+            h = getattr(lib, p + "new")(int(N), int(Ls[0]), int(Ls[1]), int(Ls[2]), float(density), int(seed))
+        if not h:
+            raise ElpError(f"{p}new/from_file failed")
+        self.h = C.c_void_p(h)
+        vals = [C.c_int() for _ in range(6)]
+        getattr(lib, p + "info")(self.h, *[C.byref(v) for v in vals])
+        self.L = (vals[0].value, vals[1].value, vals[2].value)
+        self.n_pol, self.N, self.max_pol_length = vals[3].value, vals[4].value, vals[5].value
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.prefix + "free")(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def nsite(self):
+        return self.L[0] * self.L[1] * self.L[2]
+
+    def lattice(self):
+        a = np.empty(self.nsite, dtype=self.dtype)
+        if getattr(self.lib, self.prefix + "get_lattice")(self.h, a.ctypes.data, a.nbytes):
+            raise ElpError("get_lattice failed")
+        return a
+
+    def set_lattice(self, lat):
+        a = np.ascontiguousarray(lat, dtype=self.dtype)
+        rc = getattr(self.lib, self.prefix + "set_lattice")(self.h, a.ctypes.data, a.nbytes)
+        if rc:
+            raise ElpError("set_lattice: chain recovery failed")
+
+    def write_tuv(self, path):
+        if getattr(self.lib, self.prefix + "write_tuv")(self.h, os.fsencode(path)):
+            raise ElpError("write_tuv failed")
+
+    def polymer(self, pid):
+        bonds = np.zeros(self.max_pol_length + 2, dtype=np.uint32)
+        start = np.zeros(3, dtype=np.uint32)
+        n = getattr(self.lib, self.prefix + "polymer")(self.h, int(pid), bonds.ctypes.data, len(bonds),
+                                                       start.ctypes.data)
+        if n < 0:
+            raise ElpError(f"polymer {pid} not found")
+        return tuple(int(x) for x in start), bonds[:n].copy()
+
+    def load_shift(self, tile):
+        getattr(self.lib, self.prefix + "load_shift")(self.h, *tile)
+
+    def origins(self, tile, n):
+        o = np.zeros((n, 3), dtype=np.int32)
+        getattr(self.lib, self.prefix + "origins")(self.h, *tile, int(n), o.ctypes.data)
+        return o
+
+    @property
+    def seed(self):
+        return getattr(self.lib, self.prefix + "seed")(self.h)
+
+
+class RingHost(_Host):
+    prefix = "elp_ring_host_"
+    libname = "libgpupol3_ring.so"
+    dtype = np.uint8
+
+
+class LinHost(_Host):
+    prefix = "elp_lin_host_"
+    libname = "libgpupol_lin.so"
+    dtype = np.uint16

@@ -1,0 +1,118 @@
+/*
+ * ring_session.c -- a handle-based C API over the ring host model, for
+ * callers that are not the drop-in executable (bench.py, the tests, other
+ * drivers).  It runs exactly the drop-in's start-up (SetBoxDimension,
+ * SimStateInit, GetBestPolymerPartitioning, RedistribSL: gpupol2/gpupol_init.c
+ * :83-152) and exposes the lattice, the host shift stream and the tuv writer.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "gpupol_ring.h"
+
+typedef struct elp_ring_host {
+    SimProperties sp;
+    SimState ss;
+    long cumT, cumU, cumV;
+} elp_ring_host;
+
+/* Fresh start (no files): box L^3 truncated to a multiple of 32, rings of N
+   monomers at the given density and seed.  Returns NULL on failure. */
+elp_ring_host* elp_ring_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
+    elp_ring_host* h = (elp_ring_host*)calloc(1, sizeof *h);
+    SimPropDefaultInit(&h->sp);
+    h->sp.polLength = N;
+    h->sp.density = density;
+    h->sp.seed = seed;
+    h->sp.nDevices = 1;
+    SetBoxDimension(&h->sp, LT, LU, LV);
+    SimStateInit(&h->ss, &h->sp);
+    GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
+    RedistribSL(&h->ss, &h->sp);
+    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    return h;
+}
+
+/* Start from a tuv file (resume path of LoadPolymers, without the seed bump). */
+elp_ring_host* elp_ring_host_from_file(const char* path, uint32_t seed) {
+    elp_ring_host* h = (elp_ring_host*)calloc(1, sizeof *h);
+    SimPropDefaultInit(&h->sp);
+    h->sp.seed = seed;
+    h->sp.nDevices = 1;
+    if (ReadLatticeFile(&h->ss, &h->sp, path) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    h->sp.polLength = h->ss.pol[0].length;
+    return h;
+}
+
+void elp_ring_host_free(elp_ring_host* h) {
+    if (!h) return;
+    SimStateFree(&h->ss);
+    free(h->sp.trans);
+    free(h);
+}
+
+/* box edges, number of rings, monomers per ring, maxPolLength */
+void elp_ring_host_info(const elp_ring_host* h, int* LT, int* LU, int* LV, int* nPol, int* N, int* maxPolLength) {
+    if (LT) *LT = (int)h->sp.LT;
+    if (LU) *LU = (int)h->sp.LU;
+    if (LV) *LV = (int)h->sp.LV;
+    if (nPol) *nPol = h->ss.nPol;
+    if (N) *N = h->ss.nPol ? h->ss.pol[0].length : 0;
+    if (maxPolLength) *maxPolLength = (int)h->sp.maxPolLength;
+}
+
+int elp_ring_host_get_lattice(const elp_ring_host* h, uint8_t* out, uint64_t nbytes) {
+    if (nbytes != h->sp.latSize) return -1;
+    memcpy(out, h->ss.lattice, nbytes);
+    return 0;
+}
+
+/* Replace the lattice and recover the rings from it (GetAllRingPolymers +
+   UpdatePolymerWithLabels, as after every GPU interval).  -1 if the lattice
+   does not hold exactly the labelled rings of this state. */
+int elp_ring_host_set_lattice(elp_ring_host* h, const uint8_t* in, uint64_t nbytes) {
+    if (nbytes != h->sp.latSize) return -1;
+    memcpy(h->ss.lattice, in, nbytes);
+    if (GetAllRingPolymers(&h->sp, &h->ss)) return -1;
+    return UpdatePolymerWithLabels(&h->ss);
+}
+
+int elp_ring_host_write_tuv(elp_ring_host* h, const char* path) { return WriteLatticeFile(&h->sp, &h->ss, path); }
+
+/* Bonds (incl. 0 = stored length) and start of ring `id` in tuv order.
+   Returns the ring length, or -1. */
+int elp_ring_host_polymer(const elp_ring_host* h, int id, uint32_t* bonds, int cap, uint32_t* start3) {
+    for (int p = 0; p < h->ss.nPol; p++) {
+        if (h->ss.polNumTranslate[p] != id) continue;
+        const Polymer* pol = h->ss.pol + p;
+        if (pol->length > cap) return -1;
+        for (int i = 0; i < pol->length; i++) bonds[i] = pol->bonds[i];
+        start3[0] = pol->startTUV.t; start3[1] = pol->startTUV.u; start3[2] = pol->startTUV.v;
+        return pol->length;
+    }
+    return -1;
+}
+
+/* Host shift stream of the drop-in (gpupol2/gpupol_cuda.cu:36-38,133-139):
+   elp_ring_host_load_shift draws the load-time shift; each following
+   elp_ring_host_origins call emits the tile-grid origins of n time units
+   (mod the tile edges) and advances the accumulated shift. */
+void elp_ring_host_load_shift(elp_ring_host* h, int tt, int tu, int tv) {
+    h->cumT = elp_draw_shift(&h->ss.rngState, tt);
+    h->cumU = elp_draw_shift(&h->ss.rngState, tu);
+    h->cumV = elp_draw_shift(&h->ss.rngState, tv);
+}
+
+void elp_ring_host_origins(elp_ring_host* h, int tt, int tu, int tv, int n, int32_t* out) {
+    for (int i = 0; i < n; i++) {
+        out[3 * i + 0] = (int32_t)(h->cumT % tt);
+        out[3 * i + 1] = (int32_t)(h->cumU % tu);
+        out[3 * i + 2] = (int32_t)(h->cumV % tv);
+        h->cumT += elp_draw_shift(&h->ss.rngState, tt);
+        h->cumU += elp_draw_shift(&h->ss.rngState, tu);
+        h->cumV += elp_draw_shift(&h->ss.rngState, tv);
+    }
+}
+
+uint32_t elp_ring_host_seed(const elp_ring_host* h) { return h->sp.seed; }

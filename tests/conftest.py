@@ -1,0 +1,37 @@
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "mc-polymer-simulation_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+if os.path.join(ROOT, "tests") not in sys.path:
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI of libelp.so)")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+def _ensure_built():
+    """Build only when nothing is built (the GPU box runs on prebuilt files)."""
+    need = [os.path.join(PKG, "lib", "libelp.so"), os.path.join(PKG, "lib", "libgpupol3_ring.so"),
+            os.path.join(ORACLE, "liboracle.so"), os.path.join(ORACLE, "bin", "efpol_port_ring")]
+    if all(os.path.exists(p) for p in need):
+        return
+    subprocess.run(["make", "-s", "-j8", "-C", PKG], check=True)
+    subprocess.run(["make", "-s", "-f", os.path.join(ORACLE, "Makefile")], check=True, cwd=ROOT)
+
+
+_ensure_built()
+
+
+@pytest.fixture(scope="session")
+def gpu_available():
+    import elp
+    return elp.device_count() > 0
