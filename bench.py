@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--n-iter", type=int, default=None)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-only", action="store_true",
+                    help="only run the plain copy kernel (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -71,6 +73,14 @@ def main():
         return 2
 
     import elp
+
+    if args.calib_only:
+        out = {}
+        for nb, reps in ((4 << 20, 200), (256 << 20, 20)):
+            ms = elp.bench_copy(nb, reps)
+            out[f"copy_{nb >> 20}MiB"] = {"ms": ms, "GBs_read_plus_write": 2 * nb / (ms * 1e-3) / 1e9, "reps": reps}
+        print(json.dumps(out))
+        return 0
 
     tile = tuple(int(x) for x in args.tile.split("x")) if "x" in args.tile else (int(args.tile),) * 3
     dense = args.sublattice == "dense"
@@ -104,6 +114,8 @@ def main():
     achieved = bytes_per_launch / launch_s / 1e9
     info = eng.launch_info()
 
+    copy_ms = elp.bench_copy(bytes_per_launch // 2, 200)
+    copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
     cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     cpu = None if args.no_cpu else cpu_baseline(args.L, args.N, args.density, cores, warm=300, timed=100)
 
@@ -141,6 +153,7 @@ def main():
             "kernel": "ring_time_unit",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": launch_s * 1e6,
+            "copy_kernel_same_bytes_GBs": copy_gbs,
         },
         "cpu_baseline": cpu,
     }

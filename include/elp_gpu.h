@@ -100,6 +100,12 @@ int elp_last_run_ms(elp_engine* e, float* ms);
 /* Kernel launch geometry chosen for this engine (for reports). */
 int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_unit, int* lds_bytes);
 
+/* Diagnostic (no reference counterpart): device-to-device copy of nbytes
+   (multiple of 16) with a plain streaming kernel, reps times; *ms = average
+   time per copy.  Used to calibrate HBM byte counters and to measure the
+   achievable copy bandwidth next to the published peak. */
+int elp_bench_copy(int device, uint64_t nbytes, int reps, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

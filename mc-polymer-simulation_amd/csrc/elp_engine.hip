@@ -103,10 +103,6 @@ static int create(elp_engine** out, int kind, int device, int LT, int LU, int LV
     if (LT <= 0 || LU <= 0 || LV <= 0 || LT % tt || LU % tu || LV % tv)
         return fail(ELP_E_ARG, "box %dx%dx%d not divisible by tile %dx%dx%d", LT, LU, LV, tt, tu, tv);
     if (n_iter < 0) return fail(ELP_E_ARG, "n_iter < 0");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ELP_E_NODEV, "no HIP device");
-    if (device < 0 || device >= ndev) return fail(ELP_E_ARG, "device %d out of range (%d devices)", device, ndev);
-    HIPCHK(hipSetDevice(device));
     elp_engine* e = new elp_engine();
     e->kind = kind; e->device = device;
     e->LT = LT; e->LU = LU; e->LV = LV;
@@ -123,7 +119,11 @@ static int create(elp_engine** out, int kind, int device, int LT, int LU, int LV
         return fail(ELP_E_ARG, "schedule %s tile %dx%dx%d sublattice %d threads %d is not compiled in",
                     kind == 0 ? "ring" : "linear", tt, tu, tv, sublattice, th);
     }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { delete e; return fail(ELP_E_NODEV, "no HIP device"); }
+    if (device < 0 || device >= ndev) { delete e; return fail(ELP_E_ARG, "device %d out of range (%d devices)", device, ndev); }
     hipError_t err;
+    if ((err = hipSetDevice(device)) != hipSuccess) { delete e; return hip_fail(err, "hipSetDevice"); }
     if ((err = hipMalloc(&e->d_lat, e->nbytes)) != hipSuccess) { delete e; return hip_fail(err, "hipMalloc lattice"); }
     if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
         (void)hipFree(e->d_lat); delete e; return hip_fail(err, "hipStreamCreate");
@@ -235,6 +235,40 @@ int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_
     if (threads_per_block) *threads_per_block = e->threads;
     if (blocks_per_time_unit) *blocks_per_time_unit = (e->LT / e->tt) * (e->LU / e->tu) * (e->LV / e->tv);
     if (lds_bytes) *lds_bytes = (int)(e->tt * e->tu * e->tv * (e->kind == 0 ? 1 : 2));
+    return ELP_OK;
+}
+
+__global__ void elp_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int elp_bench_copy(int device, uint64_t nbytes, int reps, float* ms) {
+    if (!ms || nbytes == 0 || nbytes % 16 || reps <= 0) return fail(ELP_E_ARG, "bad copy arguments");
+    HIPCHK(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, nbytes));
+    hipError_t err = hipMalloc(&b, nbytes);
+    if (err != hipSuccess) { (void)hipFree(a); return hip_fail(err, "hipMalloc"); }
+    (void)hipMemset(a, 1, nbytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const size_t n = nbytes / 16;
+    hipLaunchKernelGGL(elp_copy_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(elp_copy_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e1, 0);
+    err = hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return hip_fail(err, "copy kernel");
     return ELP_OK;
 }
 

@@ -1,6 +1,279 @@
-// elp_linear.hip -- gfx950 move kernel of the linear-chain ELP engine (placeholder: no schedules yet).
+// elp_linear.hip -- gfx950 move kernel of the linear-chain ELP engine.
+//
+// Same tile/sub-lattice/RNG scheme as the ring kernel (elp_ring.hip), with
+// one uint16 word per site in LDS and HBM:
+//   bits 0-3 next bond, 4-7 prev bond (bond from the previous site into this
+//   one), bit 8 stored length, bit 9 label of the monomer carrying the next
+//   bond, bit 10 label of the stored-length monomer (which precedes it).
+// Per iteration: {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490); the
+// reference runs 64 iterations per time unit (gpupol/gpupol_init.c:79).
+// The reference packs these fields into two LDS arrays with shared words
+// (gpupol/linpol.cu:97-105); here every field of a site lives in that site's
+// own word, so per-site results are the same and no read-modify-write of a
+// neighbour's word is needed.
+// Randomness: Philox4x32-10 as in the ring kernel; TransMove of iteration i
+// uses byte 1 of word i%4 of the slot stream, the coset of substep s is
+// byte s of the tile-uniform offset word.
 #include "elp_device.h"
 #include "elp_kernels.h"
+
 namespace elp {
-int lin_launch(int, int, int, bool, int, bool, int, hipStream_t, uint16_t*, const LinLaunch&) { return -1; }
+
+constexpr uint32_t kSL = 0x100u, kL0 = 0x200u, kL1 = 0x400u;
+
+template <int TT, int TU, int TV>
+struct LinTile {
+    static constexpr int NCT = TT / 4, NCU = TU / 4, NCV = TV / 4;
+    static constexpr int NCELL = NCT * NCU * NCV;
+    static constexpr int NSITE = TT * TU * TV;
+    __device__ static __forceinline__ int addr(int t, int u, int v) {
+        const int cell = (t >> 2) + NCT * ((u >> 2) + NCU * (v >> 2));
+        return cell + NCELL * ((t & 3) | ((u & 3) << 2) | ((v & 3) << 4));
+    }
+    __device__ static __forceinline__ bool inside(int t, int u, int v) {
+        return (unsigned)t < (unsigned)TT && (unsigned)u < (unsigned)TU && (unsigned)v < (unsigned)TV;
+    }
+};
+
+__device__ __forceinline__ bool bond_valid(uint32_t b) { return b != 0u && b != 3u && b != 0xcu && b != 0xfu; }
+
+// Move one stored-length unit across the next bond when exactly one end
+// holds it (gpupol/linpol.cu:314-363).
+template <class G>
+__device__ __forceinline__ void lin_diffuse_sl(uint16_t* lat, int t, int u, int v) {
+    const int a = G::addr(t, u, v);
+    uint32_t w = lat[a];
+    const uint32_t next = w & 0xfu;
+    if (!next) return;
+    const int nt = t + bdt(next), nu = u + bdu(next), nv = v + bdv(next);
+    if (!G::inside(nt, nu, nv)) return;
+    const int na = G::addr(nt, nu, nv);
+    uint32_t x = lat[na];
+    if ((w & kSL) && !(x & kSL)) {
+        const uint32_t l0 = (w >> 9) & 1u, l1 = (w >> 10) & 1u;
+        w = (w & ~(kL0 | kL1 | kSL)) | (l1 << 9);
+        x |= (l0 << 10) | kSL;
+    } else if (!(w & kSL) && (x & kSL)) {
+        const uint32_t q = (x >> 10) & 1u;
+        x &= ~(kL1 | kSL);
+        w = (w & ~kL0) | (((w >> 9) & 1u) << 10) | (q << 9) | kSL;
+    } else {
+        return;
+    }
+    lat[a] = (uint16_t)w;
+    lat[na] = (uint16_t)x;
+}
+
+// Reptation / stored-length move of one monomer (gpupol/linpol.cu:106-311):
+// with stored length at the site, split the next or prev bond (r bit 2)
+// through an empty common neighbour, or grow the chain end into an empty
+// neighbour (new bond r bits 3-6); without it, retract an end (gate r bits
+// 4-7) or fold the monomer onto its prev/next neighbour (r bit 0) as stored
+// length when the two bonds add up to one.
+template <class G>
+__device__ __forceinline__ void lin_trans_move(uint16_t* lat, int t, int u, int v, uint32_t r, const uint32_t* trans) {
+    const int a = G::addr(t, u, v);
+    uint32_t w = lat[a];
+    if (!(w & 0xffu)) return;
+    const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
+    if (w & kSL) {
+        uint32_t pb = (r >> 2) & 1u;
+        const uint32_t bond = pb ? prev : next;
+        if (bond == 0) {  // grow the chain end out of the stored length
+            pb ^= 1u;
+            const uint32_t nb = (r >> 3) & 0xfu;
+            if (!bond_valid(nb)) return;
+            const uint32_t ins = pb ? nb : (nb ^ 0xfu);
+            const int nt = t + bdt(nb), nu = u + bdu(nb), nv = v + bdv(nb);
+            if (!G::inside(nt, nu, nv)) return;
+            const int na = G::addr(nt, nu, nv);
+            uint32_t x = lat[na];
+            if (x & 0xffu) return;
+            const uint32_t lab = (w >> 9) & 3u;
+            w = (w & ~(kL0 | kL1 | kSL)) | (((lab >> pb) & 1u) << 9) | (ins << (4 * (pb ^ 1u)));
+            x |= (((lab >> (pb ^ 1u)) & 1u) << 9) | (ins << (4 * pb));
+            lat[a] = (uint16_t)w;
+            lat[na] = (uint16_t)x;
+            return;
+        }
+        const uint32_t dir = pb ? (bond ^ 0xfu) : bond;
+        const int bt = t + bdt(dir), bu = u + bdu(dir), bv = v + bdv(dir);
+        if (!G::inside(bt, bu, bv)) return;
+        uint32_t nPrev = (trans[bond >> 2] >> (4 * (2 * (bond & 3u) + (r & 1u)))) & 0xfu;
+        uint32_t nNext = bond_add((~nPrev) & 0xfu, bond);
+        if (!(r & 2u)) { const uint32_t x = nPrev; nPrev = nNext; nNext = x; }
+        const int pt = pb ? bt : t, pu = pb ? bu : u, pv = pb ? bv : v;   // old prev site
+        const int nt = pt + bdt(nPrev), nu = pu + bdu(nPrev), nv = pv + bdv(nPrev);
+        if (!G::inside(nt, nu, nv)) return;
+        const int na = G::addr(nt, nu, nv);
+        uint32_t x = lat[na];
+        if (x & 0xffu) return;
+        const int ba = G::addr(bt, bu, bv);
+        uint32_t y = lat[ba];
+        const uint32_t lab = (w >> (9 + pb)) & 1u;
+        x |= nNext | (nPrev << 4) | (lab << 9);
+        if (pb) {  // split prev bond: the stored-length monomer moves out
+            w = (w & ~(0xf0u | kSL | kL1)) | (nNext << 4);
+            y = (y & ~0xfu) | nPrev;
+        } else {   // split next bond: the bond carrier moves out
+            w = (w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9);
+            y = (y & ~0xf0u) | (nNext << 4);
+        }
+        lat[a] = (uint16_t)w;
+        lat[na] = (uint16_t)x;
+        lat[ba] = (uint16_t)y;
+        return;
+    }
+    if ((r >> 4) & 1u) return;
+    if (next == 0 || prev == 0) {  // retract a chain end onto its neighbour
+        if ((r >> 5) & 7u) return;
+        uint32_t bond, pb;
+        if (next == 0) { bond = prev ^ 0xfu; pb = 1u; }
+        else { bond = next; pb = 0u; }
+        const int nt = t + bdt(bond), nu = u + bdu(bond), nv = v + bdv(bond);
+        if (!G::inside(nt, nu, nv)) return;
+        const int na = G::addr(nt, nu, nv);
+        uint32_t x = lat[na];
+        if (x & kSL) return;
+        const uint32_t lab = (w >> 9) & 1u;
+        const uint32_t nl = (lab << (pb ^ 1u)) | (((x >> 9) & 1u) << pb);
+        w &= ~(kL0 | (0xfu << (4 * pb)));
+        x = (x & ~(kL0 | kL1 | (0xfu << (4 * (pb ^ 1u))))) | (nl << 9) | kSL;
+        lat[a] = (uint16_t)w;
+        lat[na] = (uint16_t)x;
+        return;
+    }
+    const uint32_t pb = r & 1u;
+    const uint32_t back = (~prev) & 0xfu;
+    const int pt = t + bdt(back), pu = u + bdu(back), pv = v + bdv(back);
+    const int nt = t + bdt(next), nu = u + bdu(next), nv = v + bdv(next);
+    if (!G::inside(pt, pu, pv) || !G::inside(nt, nu, nv)) return;
+    uint32_t bond;
+    if (!bond_add_valid(next, prev, bond)) return;
+    const int pa = G::addr(pt, pu, pv), na = G::addr(nt, nu, nv);
+    const int ba = pb ? pa : na;
+    if (ba == a) return;
+    uint32_t y = lat[ba];
+    if (y & kSL) return;
+    const uint32_t lab = (w >> 9) & 1u;
+    y |= kSL;
+    if (pb) y = (y & ~kL0) | (((y >> 9) & 1u) << 10) | (lab << 9);   // carrier becomes SL, monomer carries
+    else y |= lab << 10;                                               // monomer is SL before the carrier
+    lat[ba] = (uint16_t)y;
+    lat[a] = (uint16_t)(w & ~(kL0 | 0xffu));
+    lat[pa] = (uint16_t)((lat[pa] & ~0xfu) | bond);
+    lat[na] = (uint16_t)((lat[na] & ~0xf0u) | (bond << 4));
+}
+
+template <int TT, int TU, int TV, bool DENSE, int THREADS>
+__global__ __launch_bounds__(THREADS) void lin_time_unit(uint16_t* __restrict__ glat, LinLaunch p) {
+    using G = LinTile<TT, TU, TV>;
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NSLOT = G::NCELL * SPC;
+    constexpr int SPT = NSLOT / THREADS;
+    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+    extern __shared__ uint16_t tile[];
+
+    const int tid = threadIdx.x;
+    const uint32_t tileId = blockIdx.x;
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+
+    for (int i = tid; i < G::NSITE; i += THREADS) {
+        const int lt = i % TT, lu = (i / TT) % TU, lv = i / (TT * TU);
+        int gt = gt0 + lt, gu = gu0 + lu, gv = gv0 + lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        tile[G::addr(lt, lu, lv)] = glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+
+    int ct[SPT], cu[SPT], cv[SPT], jj[SPT];
+    uint32_t slot[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        slot[k] = (uint32_t)(tid + k * THREADS);
+        const int cell = (int)(slot[k] / SPC);
+        jj[k] = (int)(slot[k] % SPC);
+        ct[k] = 4 * (cell % G::NCT);
+        cu[k] = 4 * ((cell / G::NCT) % G::NCU);
+        cv[k] = 4 * (cell / (G::NCT * G::NCU));
+    }
+    __syncthreads();
+
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        uint4 sw[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+        for (int j = 0; j < nj; j++) {
+            const uint32_t offw = word_of(ow, j);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+#pragma unroll
+                for (int k = 0; k < SPT; k++) {
+                    int t, u, v;
+                    if (DENSE) {
+                        t = ct[k] + jj[k];
+                        u = cu[k] + (int)((c + 3u * (uint32_t)jj[k]) & 3u);
+                        v = cv[k] + (int)(((c >> 2) + 2u * (uint32_t)jj[k]) & 3u);
+                    } else {
+                        t = ct[k] + (int)(c & 3u);
+                        u = cu[k] + (int)((c >> 2) & 3u);
+                        v = cv[k] + (int)((c >> 4) & 3u);
+                    }
+                    if (s == 0) lin_diffuse_sl<G>(tile, t, u, v);
+                    else lin_trans_move<G>(tile, t, u, v, (word_of(sw[k], j) >> 8) & 0xffu, p.trans);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    for (int i = tid; i < G::NSITE; i += THREADS) {
+        const int lt = i % TT, lu = (i / TT) % TU, lv = i / (TT * TU);
+        int gt = gt0 + lt, gu = gu0 + lu, gv = gv0 + lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] = tile[G::addr(lt, lu, lv)];
+    }
+}
+
+#define ELP_LIN_CASE(TT, TU, TV, DENSE, TH)                                                                 \
+    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                               \
+        if (probe) return 0;                                                                                 \
+        const size_t lds = sizeof(uint16_t) * TT * TU * TV;                                                  \
+        if (lds > 65536) {                                                                                   \
+            hipError_t e = hipFuncSetAttribute((const void*)lin_time_unit<TT, TU, TV, DENSE, TH>,             \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);        \
+            if (e != hipSuccess) return (int)e;                                                              \
+        }                                                                                                    \
+        hipLaunchKernelGGL((lin_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), lds, stream, glat, \
+                           p);                                                                               \
+        return (int)hipGetLastError();                                                                       \
+    }
+
+int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
+               uint16_t* glat, const LinLaunch& p) {
+    ELP_LIN_CASE(16, 16, 16, false, 64)
+    ELP_LIN_CASE(16, 16, 16, true, 64)
+    ELP_LIN_CASE(16, 16, 16, true, 128)
+    ELP_LIN_CASE(16, 16, 16, true, 256)
+    ELP_LIN_CASE(32, 32, 32, false, 256)
+    ELP_LIN_CASE(32, 32, 32, false, 512)
+    ELP_LIN_CASE(32, 32, 32, true, 512)
+    ELP_LIN_CASE(32, 32, 32, true, 1024)
+    ELP_LIN_CASE(32, 32, 16, false, 256)
+    ELP_LIN_CASE(32, 32, 16, true, 512)
+    ELP_LIN_CASE(16, 16, 32, false, 128)
+    ELP_LIN_CASE(16, 16, 32, true, 512)
+    return -1;
+}
+
 }  // namespace elp

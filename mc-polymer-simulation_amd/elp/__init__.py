@@ -15,6 +15,7 @@ from .core import (  # noqa: F401
     Engine,
     RingHost,
     LinHost,
+    bench_copy,
     device_count,
     load_elp,
     exported_symbols,

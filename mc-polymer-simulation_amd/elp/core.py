@@ -28,7 +28,7 @@ def _load(name):
     path = os.path.join(LIB_DIR, name)
     if not os.path.exists(path):
         raise ElpError(f"{path} is not built: run `make -C {PKG_DIR}` (or __graft_entry__.build())")
-    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL if name == "libelp.so" else C.RTLD_LOCAL)
     _libs[name] = lib
     return lib
 
@@ -48,6 +48,7 @@ def load_elp():
         lib.elp_sync.argtypes = [P]
         lib.elp_last_run_ms.argtypes = [P, C.POINTER(C.c_float)]
         lib.elp_launch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        lib.elp_bench_copy.argtypes = [I, U64, I, C.POINTER(C.c_float)]
         lib._elp_typed = True
     return lib
 
@@ -74,6 +75,13 @@ def exported_symbols(path=None):
     path = path or os.path.join(LIB_DIR, "libelp.so")
     out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
     return sorted({ln.split()[-1] for ln in out.splitlines() if ln.strip()})
+
+
+def bench_copy(nbytes, reps=20, device=0):
+    """Average ms of one device-to-device copy of nbytes (plain streaming kernel)."""
+    ms = C.c_float(0)
+    _check(load_elp().elp_bench_copy(device, int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 class Engine:

@@ -1,3 +1,346 @@
-/* lin_replay.c -- ORACLE / TEST INFRASTRUCTURE ONLY (linear-chain replay; filled in with the linear engine). */
+/*
+ * lin_replay.c -- ORACLE / TEST INFRASTRUCTURE ONLY.
+ *
+ * Single-threaded C replay of the linear-chain engine's schedule, the
+ * bit-exact judge of mc-polymer-simulation_amd/csrc/elp_linear.hip.
+ *
+ * Each tile is unpacked into the reference kernel's own data structures
+ * (gpupol/linpol.cu:97-105, :446-476): a byte per site (next | prev<<4)
+ * packed 4 sites per 32-bit word, and a "slab" word per 8 sites holding two
+ * label bits per site (bits 0-15) and the stored-length bit (bits 16-23).
+ * The moves are restated operation by operation on those packed words,
+ * including their read-modify-write and re-read sequences:
+ *   TransMove  gpupol/linpol.cu:106-311
+ *   DiffuseSL  gpupol/linpol.cu:314-363
+ *   EdgeOutside (tile faces) gpupol/linpol.cu:365-386
+ * Site index inside a tile: bits 0-2 = (t&1, u&1, v&1), so the sites that
+ * share a byte word / slab word are the same as in the reference; bits 3-5
+ * the second bits of (t,u,v), higher bits the 4x4x4 cell.
+ * Schedule and Philox streams as in ring_replay.c, with substeps
+ * {DiffuseSL, TransMove} and TransMove bits = byte 1 of the slot word.
+ *
+ * Pinned by: Philox known-answer vectors; slot-order independence; chain
+ * invariants (monomer count, one start and one end per chain, label
+ * sequence of every chain unchanged, host recovery of every id).
+ */
 #include <stdint.h>
-int orc_lin_placeholder(void) { return 0; }
+#include <stdlib.h>
+#include <string.h>
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+typedef struct {
+    int T, U, V, nct, ncu;
+    uint32_t* lattice;   /* 4 sites per word */
+    uint32_t* slab;      /* 8 sites per word */
+    uint32_t tr[4];
+} LTile;
+
+static uint32_t lidx(const LTile* X, int t, int u, int v) {
+    uint32_t cell = (uint32_t)((t >> 2) + X->nct * ((u >> 2) + X->ncu * (v >> 2)));
+    return (uint32_t)(t & 1) | (uint32_t)(u & 1) << 1 | (uint32_t)(v & 1) << 2 | (uint32_t)((t >> 1) & 1) << 3 |
+           (uint32_t)((u >> 1) & 1) << 4 | (uint32_t)((v >> 1) & 1) << 5 | cell << 6;
+}
+static void lcoord(const LTile* X, uint32_t s, int* t, int* u, int* v) {
+    uint32_t cell = s >> 6;
+    *t = 4 * (int)(cell % X->nct) + (int)(s & 1) + 2 * (int)((s >> 3) & 1);
+    *u = 4 * (int)((cell / X->nct) % X->ncu) + (int)((s >> 1) & 1) + 2 * (int)((s >> 4) & 1);
+    *v = 4 * (int)(cell / (X->nct * X->ncu)) + (int)((s >> 2) & 1) + 2 * (int)((s >> 5) & 1);
+}
+static int sdt(uint32_t b) { return (int)(b & 1) - (int)(b >> 3); }
+static int sdu(uint32_t b) { return (int)((b >> 1) & 1) - (int)(b >> 3); }
+static int sdv(uint32_t b) { return (int)((b >> 2) & 1) - (int)(b >> 3); }
+/* EdgeOutside: would site+bond leave the tile? */
+static int edge_outside(const LTile* X, uint32_t s, uint32_t b) {
+    int t, u, v;
+    lcoord(X, s, &t, &u, &v);
+    t += sdt(b); u += sdu(b); v += sdv(b);
+    return t < 0 || t >= X->T || u < 0 || u >= X->U || v < 0 || v >= X->V;
+}
+/* AddUnitToSite (caller has checked edge_outside) */
+static uint32_t add_unit(const LTile* X, uint32_t b, uint32_t s) {
+    int t, u, v;
+    lcoord(X, s, &t, &u, &v);
+    return lidx(X, t + sdt(b), u + sdu(b), v + sdv(b));
+}
+static uint32_t uadd(uint32_t a, uint32_t b) { return ((a | b) == 0xf) ? (a & b) : (a | b); }
+static int uvalid_add(uint32_t a, uint32_t b, uint32_t* c) {
+    if ((a | b) != 0xf && (a & b)) return 0;
+    uint32_t r = uadd(a, b);
+    *c = r;
+    return (r == 0x3 || r == 0xc) ? 0 : 1;
+}
+static int unit_ok(uint32_t u) { return !(u == 0x0 || u == 0x3 || u == 0xc || u == 0xf); }
+
+static void trans_move(LTile* X, uint32_t site, uint32_t rnd) {
+    uint32_t* lattice = X->lattice;
+    uint32_t* slab = X->slab;
+    uint32_t next, prev, bond, temp, oldNextSite, oldPrevSite, bondSite, newSite, nNext, nPrev, prevBond;
+    uint32_t latSiteComplete, nextPrev, slabSite, latNew, latBond, latPrev, latNext, newBond, slabNew, insBond,
+        slabBond, label;
+    latSiteComplete = lattice[site / 4];
+    nextPrev = (latSiteComplete >> (8 * (site & 0x3))) & 0xff;
+    prev = nextPrev >> 4;
+    next = nextPrev & 0xf;
+    slabSite = slab[site / 8];
+    if (!nextPrev) return;
+    if ((slabSite >> (16 | (site & 0x7))) & 0x1) {
+        prevBond = (rnd & 0x4) >> 2;
+        bond = (nextPrev >> (prevBond * 4)) & 0xf;
+        if (bond == 0) {
+            prevBond ^= 0x1;
+            newBond = (rnd >> 3) & 0xf;
+            insBond = newBond ^ (0xf * (prevBond ^ 0x1));
+            if (!unit_ok(newBond)) return;
+            if (edge_outside(X, site, newBond)) return;
+            newSite = add_unit(X, newBond, site);
+            latNew = lattice[newSite / 4];
+            if (latNew & (0xffu << ((newSite & 0x3) * 8))) return;
+            label = (slabSite >> ((site & 0x7) * 2)) & 0x3;
+            slabSite &= ~(0x3u << ((site & 0x7) * 2));
+            slabSite |= ((label >> prevBond) & 0x1) << ((site & 0x7) * 2);
+            slabSite &= ~(1u << (16 | (site & 0x7)));
+            slab[site / 8] = slabSite;
+            slabNew = slab[newSite / 8];
+            slabNew |= ((label >> (prevBond ^ 0x1)) & 0x1) << ((newSite & 0x7) * 2);
+            slab[newSite / 8] = slabNew;
+            latSiteComplete |= insBond << ((site & 0x3) * 8 + 4 * (prevBond ^ 0x1));
+            lattice[site / 4] = latSiteComplete;
+            if (site / 4 == newSite / 4) latNew = lattice[newSite / 4];
+            latNew |= insBond << ((newSite & 0x3) * 8 + 4 * prevBond);
+            lattice[newSite / 4] = latNew;
+            return;
+        }
+        if (edge_outside(X, site, bond ^ (0xf * prevBond))) return;
+        nPrev = (X->tr[bond / 4] >> (4 * (2 * (bond % 4) + (rnd & 0x1)))) & 0xf;
+        nNext = uadd((~nPrev) & 0xf, bond);
+        temp = nPrev;
+        nPrev = (rnd & 0x2) ? nPrev : nNext;
+        nNext = (rnd & 0x2) ? nNext : temp;
+        bondSite = add_unit(X, bond ^ (0xf * prevBond), site);
+        oldNextSite = prevBond ? site : bondSite;
+        oldPrevSite = prevBond ? bondSite : site;
+        (void)oldNextSite;
+        if (edge_outside(X, oldPrevSite, nPrev)) return;
+        newSite = add_unit(X, nPrev, oldPrevSite);
+        latNew = lattice[newSite / 4];
+        if (latNew & (0xffu << ((newSite & 0x3) * 8))) return;
+        latNew |= (nNext | (nPrev << 4)) << ((newSite & 0x3) * 8);
+        latSiteComplete &= ~(0xfu << ((site & 0x3) * 8 + 4 * prevBond));
+        latSiteComplete |= (prevBond ? (nNext << 4) : nPrev) << ((site & 0x3) * 8);
+        if (newSite / 4 == site / 4) {
+            lattice[site / 4] = (latSiteComplete & (0xffu << ((site & 0x3) * 8))) | (latNew & ~(0xffu << ((site & 0x3) * 8)));
+        } else {
+            lattice[site / 4] = latSiteComplete;
+            lattice[newSite / 4] = latNew;
+        }
+        latBond = lattice[bondSite / 4];
+        latBond &= ~(0xfu << ((bondSite & 0x3) * 8 + 4 * (prevBond ^ 0x1)));
+        latBond |= (prevBond ? nPrev : (nNext << 4)) << ((bondSite & 0x3) * 8);
+        lattice[bondSite / 4] = latBond;
+        label = (slabSite >> (((site & 0x7) * 2) | prevBond)) & 0x1;
+        slabSite &= ~((1u << ((site & 0x7) | 16)) | (1u << (((site & 0x7) * 2) | prevBond)));
+        if (!prevBond) {
+            slabSite |= (slabSite & (1u << ((site & 0x7) * 2 + 1))) >> 1;
+            slabSite &= ~(1u << ((site & 0x7) * 2 + 1));
+        }
+        slab[site / 8] = slabSite;
+        slab[newSite / 8] |= label << ((newSite & 0x7) * 2);
+    } else {
+        if (((rnd >> 4) & 0x1) != 0) return;
+        if (next == 0 || prev == 0) {
+            if (((rnd >> 5) & 0x7) != 0) return;
+            if (next == 0) { bond = prev ^ 0xf; prevBond = 1; }
+            else { bond = next; prevBond = 0; }
+            if (edge_outside(X, site, bond)) return;
+            newSite = add_unit(X, bond, site);
+            slabBond = slab[newSite / 8];
+            if (slabBond & (1u << (16 | (newSite & 0x7)))) return;
+            label = (slabSite >> ((site & 0x7) * 2)) & 0x1;
+            slabSite &= ~(1u << ((site & 0x7) * 2));
+            slab[site / 8] = slabSite;
+            if (newSite / 8 == site / 8) slabBond = slab[newSite / 8];
+            label <<= prevBond ^ 0x1;
+            label |= ((slabBond >> ((newSite & 0x7) * 2)) & 0x1) << prevBond;
+            slabBond &= ~(0x3u << ((newSite & 0x7) * 2));
+            slabBond |= label << ((newSite & 0x7) * 2);
+            slabBond |= (1u << (16 | (newSite & 0x7)));
+            slab[newSite / 8] = slabBond;
+            latSiteComplete &= ~(0xfu << (8 * (site & 0x3) + 4 * prevBond));
+            lattice[site / 4] = latSiteComplete;
+            latNew = lattice[newSite / 4];
+            latNew &= ~(0xfu << ((newSite & 0x3) * 8 + 4 * (prevBond ^ 0x1)));
+            lattice[newSite / 4] = latNew;
+            return;
+        }
+        prevBond = rnd & 0x1;
+        if (edge_outside(X, site, (~prev) & 0xf) | edge_outside(X, site, next)) return;
+        if (!uvalid_add(next, prev, &bond)) return;
+        oldNextSite = add_unit(X, next, site);
+        oldPrevSite = add_unit(X, (~prev) & 0xf, site);
+        bondSite = prevBond ? oldPrevSite : oldNextSite;
+        if (bondSite == site) return;
+        slabBond = slab[bondSite / 8];
+        if (slabBond & (1u << ((bondSite & 0x7) | 16))) return;
+        slab[bondSite / 8] = slabBond | (1u << ((bondSite & 0x7) | 16));
+        slabSite = slab[site / 8];
+        label = (slabSite >> (2 * (site & 0x7))) & 0x1;
+        slab[site / 8] = slabSite & ~(1u << (2 * (site & 0x7)));
+        slabBond = slab[bondSite / 8];
+        if (prevBond) {
+            slabBond |= (slabBond & (1u << (2 * (bondSite & 0x7)))) << 1;
+            slabBond &= ~(1u << (2 * (bondSite & 0x7)));
+        }
+        slab[bondSite / 8] = slabBond | (label << ((2 * (bondSite & 0x7)) | (prevBond ^ 0x1)));
+        lattice[site / 4] = latSiteComplete & ~(0xffu << ((site & 0x3) * 8));
+        latPrev = lattice[oldPrevSite / 4];
+        latPrev &= ~(0xfu << ((oldPrevSite % 4) * 8));
+        latPrev |= (bond << ((oldPrevSite % 4) * 8));
+        lattice[oldPrevSite / 4] = latPrev;
+        latNext = lattice[oldNextSite / 4];
+        latNext &= ~(0xf0u << ((oldNextSite % 4) * 8));
+        latNext |= (bond << ((oldNextSite % 4) * 8 + 4));
+        lattice[oldNextSite / 4] = latNext;
+    }
+}
+
+static void diffuse_sl(LTile* X, uint32_t site) {
+    uint32_t* lattice = X->lattice;
+    uint32_t* slab = X->slab;
+    uint32_t next, nextSite, tsl, bit, nBit, lBit, lnBit, slabSite, slabNext, label;
+    next = (lattice[site >> 2] >> (8 * (site & 0x3))) & 0xff;
+    next &= 0xf;
+    if (!next || edge_outside(X, site, next)) return;
+    nextSite = add_unit(X, next, site);
+    slabSite = slab[site / 8];
+    slabNext = slab[nextSite / 8];
+    bit = 16 | (site & 0x7);
+    nBit = 16 | (nextSite & 0x7);
+    lBit = (site & 0x7) * 2;
+    lnBit = (nextSite & 0x7) * 2;
+    if ((slabSite & (1u << bit)) && !(slabNext & (1u << nBit))) {
+        label = slabSite & (0x3u << lBit);
+        slabSite &= ~(0x3u << lBit);
+        slabSite |= (label & (0x2u << lBit)) >> 1;
+        label = (label >> lBit) & 0x1;
+        slabNext |= label << (lnBit + 1);
+    } else if (!(slabSite & (1u << bit)) && (slabNext & (1u << nBit))) {
+        label = (slabNext >> (lnBit + 1)) & 0x1;
+        slabNext &= ~(1u << (lnBit + 1));
+        slabSite |= (slabSite & (1u << lBit)) << 1;
+        slabSite &= ~(1u << lBit);
+        slabSite |= label << lBit;
+    } else {
+        return;
+    }
+    tsl = ((slabSite >> bit) & 0x1) << nBit;
+    slabSite &= ~(1u << bit);
+    slabSite |= ((slabNext >> nBit) & 0x1) << bit;
+    slabNext &= ~(1u << nBit);
+    slabNext |= tsl;
+    if (site / 8 == nextSite / 8) {
+        slab[site / 8] = (slabSite & ((0x3u << lBit) | (0x1u << bit))) | (slabNext & ~((0x3u << lBit) | (0x1u << bit)));
+    } else {
+        slab[site / 8] = slabSite;
+        slab[nextSite / 8] = slabNext;
+    }
+}
+
+static void gpu_trans(uint32_t tr[4]) {
+    int nf[16] = {0};
+    memset(tr, 0, 4 * sizeof(uint32_t));
+    for (uint32_t a = 1; a < 15; a++) {
+        if (!unit_ok(a)) continue;
+        for (uint32_t b = a + 1; b < 15; b++) {
+            if (!unit_ok(b)) continue;
+            if ((a | b) != 0xf && (a & b)) continue;
+            uint32_t s = uadd(a, b);
+            if (!unit_ok(s)) continue;
+            tr[s / 4] |= a << (8 * (s % 4) + 4 * nf[s]);
+            nf[s]++;
+        }
+    }
+}
+
+static uint32_t philox_word(uint32_t tau, uint32_t tile, uint32_t lane, uint32_t iter, uint32_t seed, uint32_t stream) {
+    uint32_t ctr[4] = {tau, tile, lane, iter >> 2}, key[2] = {seed, stream}, out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return out[iter & 3];
+}
+
+int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, int sublat, int n_iter, int n_time,
+                const int32_t* origins, uint64_t tau0, uint32_t seed, uint32_t stream, int order) {
+    if (TT % 4 || TU % 4 || TV % 4 || LT % TT || LU % TU || LV % TV) return -1;
+    LTile X;
+    X.T = TT; X.U = TU; X.V = TV; X.nct = TT / 4; X.ncu = TU / 4;
+    int nsite = TT * TU * TV;
+    X.lattice = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 4));
+    X.slab = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 8));
+    gpu_trans(X.tr);
+    int nT = LT / TT, nU = LU / TU, nV = LV / TV;
+    int nslot = (TT / 4) * (TU / 4) * (TV / 4) * (sublat ? 4 : 1);
+    int* perm = (int*)malloc(sizeof(int) * nslot);
+    uint32_t cmask = sublat ? 0xf : 0x3f;
+    for (int i = 0; i < n_time; i++) {
+        int ot = origins[3 * i], ou = origins[3 * i + 1], ov = origins[3 * i + 2];
+        uint32_t tau = (uint32_t)(tau0 + (uint64_t)i);
+        for (int kv = 0; kv < nV; kv++)
+            for (int ku = 0; ku < nU; ku++)
+                for (int kt = 0; kt < nT; kt++) {
+                    uint32_t tile = (uint32_t)(kt + nT * (ku + nU * kv));
+                    memset(X.lattice, 0, sizeof(uint32_t) * (nsite / 4));
+                    memset(X.slab, 0, sizeof(uint32_t) * (nsite / 8));
+                    for (int v = 0; v < TV; v++)
+                        for (int u = 0; u < TU; u++)
+                            for (int t = 0; t < TT; t++) {
+                                size_t g = (size_t)((ot + kt * TT + t) % LT) +
+                                           (size_t)LT * ((size_t)((ou + ku * TU + u) % LU) + (size_t)LU * ((ov + kv * TV + v) % LV));
+                                uint32_t w = lat[g], s = lidx(&X, t, u, v);
+                                X.lattice[s / 4] |= (w & 0xff) << (8 * (s % 4));
+                                X.slab[s / 8] |= ((w >> 8) & 1) << (16 | (s & 7));
+                                X.slab[s / 8] |= ((w >> 9) & 3) << ((s & 7) * 2);
+                            }
+                    for (int it = 0; it < n_iter; it++) {
+                        uint32_t ow = philox_word(tau, tile, 0xFFFFFFFFu, (uint32_t)it, seed, stream);
+                        for (int s = 0; s < 2; s++) {
+                            uint32_t c = (ow >> (8 * s)) & cmask;
+                            for (int k = 0; k < nslot; k++) perm[k] = (order == 1) ? nslot - 1 - k : k;
+                            if (order == 2) {
+                                uint32_t x = 2654435761u * (uint32_t)(it * 2 + s + 1) + tile;
+                                for (int k = nslot - 1; k > 0; k--) {
+                                    x = x * 1664525u + 1013904223u;
+                                    int j = (int)((x >> 8) % (uint32_t)(k + 1));
+                                    int tmp = perm[k]; perm[k] = perm[j]; perm[j] = tmp;
+                                }
+                            }
+                            for (int kk = 0; kk < nslot; kk++) {
+                                int slot = perm[kk];
+                                int cell = sublat ? slot / 4 : slot, j = sublat ? slot % 4 : 0;
+                                int t = 4 * (cell % X.nct), u = 4 * ((cell / X.nct) % X.ncu), v = 4 * (cell / (X.nct * X.ncu));
+                                if (sublat) { t += j; u += (int)((c + 3u * j) % 4); v += (int)(((c >> 2) + 2u * j) % 4); }
+                                else { t += (int)(c & 3); u += (int)((c >> 2) & 3); v += (int)((c >> 4) & 3); }
+                                uint32_t site = lidx(&X, t, u, v);
+                                if (s == 0) diffuse_sl(&X, site);
+                                else trans_move(&X, site,
+                                                (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff);
+                            }
+                        }
+                    }
+                    for (int v = 0; v < TV; v++)
+                        for (int u = 0; u < TU; u++)
+                            for (int t = 0; t < TT; t++) {
+                                size_t g = (size_t)((ot + kt * TT + t) % LT) +
+                                           (size_t)LT * ((size_t)((ou + ku * TU + u) % LU) + (size_t)LU * ((ov + kv * TV + v) % LV));
+                                uint32_t s = lidx(&X, t, u, v);
+                                uint32_t w = (X.lattice[s / 4] >> (8 * (s % 4))) & 0xff;
+                                w |= ((X.slab[s / 8] >> (16 | (s & 7))) & 1) << 8;
+                                w |= ((X.slab[s / 8] >> ((s & 7) * 2)) & 3) << 9;
+                                lat[g] = (uint16_t)w;
+                            }
+                }
+    }
+    free(perm);
+    free(X.lattice);
+    free(X.slab);
+    return 0;
+}

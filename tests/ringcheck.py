@@ -92,3 +92,69 @@ def replay_ring(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, ord
                                int(tau0), int(seed), int(stream), int(order))
     assert rc == 0
     return lat
+
+
+# ---------------- linear chains (uint16 site words) ----------------
+def lin_site_invariants(lat, L):
+    """Encoding rules and bond consistency of a linear-chain lattice:
+    next(x) = b  <=>  prev(x+b) = b.  Returns (monomers, chains)."""
+    lat = np.asarray(lat, dtype=np.uint16)
+    LT, LU, LV = L
+    nxt = (lat & 0xF).astype(np.int64)
+    prv = ((lat >> 4) & 0xF).astype(np.int64)
+    sl = (lat >> 8) & 1
+    occ = (lat & 0xFF) != 0
+    assert not np.any(lat & 0xF800), "unused bits set"
+    assert np.all(VALID[nxt[nxt != 0]]) and np.all(VALID[prv[prv != 0]]), "invalid bond code"
+    assert not np.any((sl == 1) & ~occ), "stored length on an empty site"
+    assert not np.any(((lat >> 10) & 1) & (sl == 0)), "SL label without stored length"
+    idx = np.arange(lat.size)
+    t, u, v = idx % LT, (idx // LT) % LU, idx // (LT * LU)
+    has = nxt != 0
+    tg = ((t + DT[nxt]) % LT) + LT * (((u + DU[nxt]) % LU) + LU * ((v + DV[nxt]) % LV))
+    assert np.array_equal(prv[tg[has]], nxt[has]), "next/prev mismatch"
+    assert int(np.count_nonzero(prv != 0)) == int(np.count_nonzero(has)), "dangling prev bonds"
+    starts = int(np.count_nonzero(occ & (prv == 0)))
+    ends = int(np.count_nonzero(occ & (nxt == 0)))
+    assert starts == ends
+    return int(occ.sum() + sl.sum()), starts
+
+
+def lin_label_sequences(lat, L):
+    """Label sequence of every chain read from its start (monomer 0)."""
+    lat = np.asarray(lat, dtype=np.uint16)
+    LT, LU, LV = L
+    out = []
+    for s0 in np.flatnonzero(((lat & 0xFF) != 0) & (((lat >> 4) & 0xF) == 0)):
+        seq = []
+        s = int(s0)
+        for _ in range(lat.size + 1):
+            w = int(lat[s])
+            if (w >> 8) & 1:
+                seq.append((w >> 10) & 1)
+            nx = w & 0xF
+            if nx == 0:
+                seq.append(-1)  # end monomer: its label bit is not stored at start-up
+                break
+            seq.append((w >> 9) & 1)
+            t, u, v = s % LT, (s // LT) % LU, s // (LT * LU)
+            s = ((t + DT[nx]) % LT) + LT * (((u + DU[nx]) % LU) + LU * ((v + DV[nx]) % LV))
+        out.append("".join("x" if b < 0 else str(b) for b in seq))
+    return sorted(out)
+
+
+_lin_typed = False
+
+
+def replay_lin(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0):
+    global _lin_typed
+    lib = oracle()
+    if not _lin_typed:
+        lib.orc_lin_run.argtypes = [C.c_void_p] + [C.c_int] * 9 + [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]
+        _lin_typed = True
+    lat = np.ascontiguousarray(lat, dtype=np.uint16).copy()
+    o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+    rc = lib.orc_lin_run(lat.ctypes.data, *L, *tile, int(sublat), int(n_iter), len(o), o.ctypes.data, int(tau0),
+                         int(seed), int(stream), int(order))
+    assert rc == 0
+    return lat

@@ -1,0 +1,50 @@
+"""The C-ABI libraries load and export every declared entry point (no GPU calls)."""
+import ctypes as C
+import os
+import subprocess
+
+import elp
+
+LIB = elp.LIB_DIR
+GPULIB_RING = ["GPULibInit", "GPULibLoadBuffers", "GPULibRun", "GPULibEqRun", "GPULoadLattices", "GPULatticeToCPU",
+               "GPULibRelease"]
+GPULIB_LIN = ["GPULibInit", "GPULibLoadBuffers", "GPULibRun", "GPULibRelease", "GPULatticeToCPU"]
+HOST = ["new", "from_file", "free", "info", "get_lattice", "set_lattice", "write_tuv", "polymer", "load_shift",
+        "origins", "seed"]
+
+
+def test_libelp_exports_every_declared_symbol():
+    declared = elp.declared_symbols()
+    assert "elp_run" in declared and "elp_ring_create" in declared and len(declared) >= 12
+    exported = set(elp.exported_symbols())
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    lib = elp.load_elp()
+    for s in declared:
+        getattr(lib, s)
+
+
+def test_host_libraries_export_gpulib_boundary():
+    ring = set(elp.exported_symbols(os.path.join(LIB, "libgpupol3_ring.so")))
+    lin = set(elp.exported_symbols(os.path.join(LIB, "libgpupol_lin.so")))
+    assert all(s in ring for s in GPULIB_RING)
+    assert all(s in lin for s in GPULIB_LIN)
+    assert all("elp_ring_host_" + s in ring for s in HOST)
+    assert all("elp_lin_host_" + s in lin for s in HOST)
+
+
+def test_error_paths_without_gpu_calls():
+    """Argument validation happens before any device call."""
+    lib = elp.load_elp()
+    h = C.c_void_p()
+    assert lib.elp_ring_create(C.byref(h), 0, 96, 96, 96, 24, 24, 24, 0, 51) == -1   # schedule not compiled in
+    assert b"" != lib.elp_last_error()
+    assert lib.elp_ring_create(C.byref(h), 0, 100, 96, 96, 16, 16, 16, 0, 51) == -1  # box not divisible
+    assert b"divisible" in lib.elp_last_error()
+    assert lib.elp_run(None, 1, None, 0, 0, 0) == -1
+
+
+def test_drop_in_executables_print_usage():
+    for exe in ("gpupol3_cuda_ring", "gpupol_cuda_lin"):
+        r = subprocess.run([os.path.join(elp.PKG_DIR, "bin", exe)], capture_output=True, text=True)
+        assert r.returncode == 2 and "usage" in r.stderr

@@ -1,0 +1,51 @@
+"""GPU parity of the linear-chain engine against the packed-layout C replay."""
+import sys
+
+import numpy as np
+import pytest
+
+import elp
+from ringcheck import lin_label_sequences, lin_site_invariants, replay_lin
+
+pytestmark = pytest.mark.gpu
+
+SCHEDULES = [
+    ((32, 32, 32), elp.SUBLAT_CUBIC, 64),
+    ((16, 16, 16), elp.SUBLAT_CUBIC, 64),
+    ((16, 16, 16), elp.SUBLAT_DENSE, 16),
+    ((32, 32, 16), elp.SUBLAT_DENSE, 16),
+]
+
+
+@pytest.mark.parametrize("tile,sublat,n_iter", SCHEDULES)
+@pytest.mark.parametrize("N,seed", [(200, 0), (50, 1)])
+def test_lin_bit_exact_vs_replay(tile, sublat, n_iter, N, seed):
+    print(f"[lin parity] tile={tile} sublat={sublat} n_iter={n_iter} N={N} seed={seed} L=32", flush=True)
+    h = elp.LinHost(N=N, L=32, density=1.2, seed=seed)
+    lat0 = h.lattice()
+    org = h.origins(tile, 10)
+    eng = elp.Engine("linear", h.L, tile, sublat, n_iter)
+    eng.upload(lat0)
+    eng.run(org[:4], tau0=0, seed=h.seed)
+    eng.run(org[4:], tau0=4, seed=h.seed)
+    got = eng.download()
+    want = replay_lin(lat0, h.L, tile, sublat, n_iter, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > 1000
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
+
+
+def test_lin_full_size_invariants():
+    """Config of the linear target: L=128, N=500 (5033 chains), 100 time units."""
+    print("[lin full size] L=128 N=500", flush=True)
+    sys.stdout.flush()
+    h = elp.LinHost(N=500, L=128, density=1.2, seed=5)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    org = h.origins(tile, 100)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    assert lin_site_invariants(got, h.L) == (5033 * 500, 5033)
+    assert np.count_nonzero(got != lat0) > lat0.size // 4
+    h.set_lattice(got)
