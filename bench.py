@@ -69,7 +69,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(independent replicas shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(independent replicas run one engine per GPU with no exchange)", file=sys.stderr)
         return 2
 
     import elp
