@@ -29,6 +29,17 @@ __device__ __forceinline__ uint32_t word_of(const uint4& w, int j) {
 // the slot index (always < 2^31) there instead.
 constexpr uint32_t kOffsetStream = 0xFFFFFFFFu;
 
+// Logical tile of a workgroup.  Workgroups are dispatched round-robin over
+// the 8 XCDs (block b -> XCD b % 8), each with its own L2.  Giving XCD x the
+// contiguous tile range [x*n/8, (x+1)*n/8) (t fastest) keeps the tiles that
+// share 128-byte lines of the row-major lattice on one L2, so each line is
+// fetched and written back once per time unit.  Tiles are independent and
+// the random streams are keyed by the logical tile, so the mapping changes
+// no result.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    return (n & 7u) ? b : (b & 7u) * (n >> 3) + (b >> 3);
+}
+
 // 4-bit bond code (t,u,v,w) -> displacement (t-w, u-w, v-w)
 __device__ __forceinline__ int bdt(uint32_t b) { return (int)(b & 1u) - (int)(b >> 3); }
 __device__ __forceinline__ int bdu(uint32_t b) { return (int)((b >> 1) & 1u) - (int)(b >> 3); }

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
     __shared__ uint8_t tile[G::NBYTE];
 
     const int tid = threadIdx.x;
-    const uint32_t tileId = blockIdx.x;
+    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
     const int kt = (int)(tileId % (uint32_t)p.ntile_t);
     const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
     const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));

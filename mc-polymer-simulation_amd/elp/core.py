@@ -163,6 +163,7 @@ class _Host:
             getattr(lib, p + "polymer").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
             getattr(lib, p + "load_shift").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
             getattr(lib, p + "origins").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+            getattr(lib, p + "seeds").argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
             getattr(lib, p + "seed").restype = C.c_uint32
             getattr(lib, p + "seed").argtypes = [C.c_void_p]
             lib._host_typed = True
@@ -222,6 +223,14 @@ class _Host:
         o = np.zeros((n, 3), dtype=np.int32)
         getattr(self.lib, self.prefix + "origins")(self.h, *tile, int(n), o.ctypes.data)
         return o
+
+    def seed_table(self):
+        """Per-lane seed table of the reference start-up (uint32[nwg*512*8])."""
+        n = (self.L[0] // 32) * (self.L[1] // 32) * (self.L[2] // 32) * 512 * 8
+        a = np.zeros(n, dtype=np.uint32)
+        if getattr(self.lib, self.prefix + "seeds")(self.h, a.ctypes.data, n):
+            raise ElpError("seed table size mismatch")
+        return a
 
     @property
     def seed(self):

@@ -180,3 +180,12 @@ void elp_lin_host_origins(elp_lin_host* h, int tt, int tu, int tv, int n, int32_
 }
 
 uint32_t elp_lin_host_seed(const elp_lin_host* h) { return h->sp.seed; }
+
+/* per-lane seed table of SimStateInit (gpupol/gpupol_init.c:23-34); see
+   elp_ring_host_seeds */
+int elp_lin_host_seeds(const elp_lin_host* h, uint32_t* out, uint64_t n) {
+    uint64_t need = (uint64_t)h->sp.nwg * h->sp.ws * 2 * h->sp.R;
+    if (n != need) return -1;
+    memcpy(out, h->ss.seeds, need * sizeof(uint32_t));
+    return 0;
+}

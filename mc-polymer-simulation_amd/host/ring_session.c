@@ -116,3 +116,14 @@ void elp_ring_host_origins(elp_ring_host* h, int tt, int tu, int tv, int n, int3
 }
 
 uint32_t elp_ring_host_seed(const elp_ring_host* h) { return h->sp.seed; }
+
+/* The per-lane seed table drawn by SimStateInit (gpupol2/gpupol_init.c:22-33):
+   (nwg * 512 * 8) uint32, lane l of work-group w at 8*(l + 512*w): 4 words of
+   its own state, then 4 words shared by the work-group.  The counter-based
+   engine does not use it; it is exposed for reference-kernel comparisons. */
+int elp_ring_host_seeds(const elp_ring_host* h, uint32_t* out, uint64_t n) {
+    uint64_t need = (uint64_t)h->sp.nwg * h->sp.ws * 2 * h->sp.R;
+    if (n != need) return -1;
+    memcpy(out, h->ss.seeds, need * sizeof(uint32_t));
+    return 0;
+}

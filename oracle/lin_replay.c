@@ -72,7 +72,20 @@ static int uvalid_add(uint32_t a, uint32_t b, uint32_t* c) {
 }
 static int unit_ok(uint32_t u) { return !(u == 0x0 || u == 0x3 || u == 0xc || u == 0xf); }
 
-static void trans_move(LTile* X, uint32_t site, uint32_t rnd) {
+/* per-move random bits: the slot's fixed byte (engine mode) or a draw from
+   the lane's LFSR113 state where the reference kernel calls Rng4
+   (gpupol/linpol.cu:48-59, :123) */
+typedef struct { int ref; uint32_t fixed; uint32_t* st; } LDraw;
+static uint32_t lfsr113_dev(uint32_t* s) {
+    uint32_t b;
+    b = ((s[0] << 6) ^ s[0]) >> 13;  s[0] = ((s[0] & 4294967294u) << 18) ^ b;
+    b = ((s[1] << 2) ^ s[1]) >> 27;  s[1] = ((s[1] & 4294967288u) << 2) ^ b;
+    b = ((s[2] << 13) ^ s[2]) >> 21; s[2] = ((s[2] & 4294967280u) << 7) ^ b;
+    b = ((s[3] << 3) ^ s[3]) >> 12;  s[3] = ((s[3] & 4294967168u) << 13) ^ b;
+    return s[0] ^ s[1] ^ s[2] ^ s[3];
+}
+
+static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
     uint32_t* lattice = X->lattice;
     uint32_t* slab = X->slab;
     uint32_t next, prev, bond, temp, oldNextSite, oldPrevSite, bondSite, newSite, nNext, nPrev, prevBond;
@@ -84,6 +97,7 @@ static void trans_move(LTile* X, uint32_t site, uint32_t rnd) {
     next = nextPrev & 0xf;
     slabSite = slab[site / 8];
     if (!nextPrev) return;
+    uint32_t rnd = rd->ref ? lfsr113_dev(rd->st) : rd->fixed;
     if ((slabSite >> (16 | (site & 0x7))) & 0x1) {
         prevBond = (rnd & 0x4) >> 2;
         bond = (nextPrev >> (prevBond * 4)) & 0xf;
@@ -321,8 +335,10 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
                                 else { t += (int)(c & 3); u += (int)((c >> 2) & 3); v += (int)((c >> 4) & 3); }
                                 uint32_t site = lidx(&X, t, u, v);
                                 if (s == 0) diffuse_sl(&X, site);
-                                else trans_move(&X, site,
-                                                (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff);
+                                else {
+                                    LDraw rd = {0, (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff, NULL};
+                                    trans_move(&X, site, &rd);
+                                }
                             }
                         }
                     }
@@ -340,6 +356,74 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
                 }
     }
     free(perm);
+    free(X.lattice);
+    free(X.slab);
+    return 0;
+}
+
+/*
+ * Reference mode: the schedule of gpupol/linpol.cu itself -- 32^3 tiles at a
+ * fresh offset per time unit (origins), one lane per 4x4x4 cell (lane bits
+ * t:0-2 u:3-5 v:6-8, :440-444), per-lane LFSR113 states rngp/rngl from and
+ * back to the seed table (gpupol/gpupol_init.c:23-34); per iteration one rngl
+ * draw masked by 0x88f selects the DiffuseSL site and a second one the
+ * TransMove site (:483-490).  Judge: the reference kernel compiled
+ * unmodified (oracle/refkernel_lin.hip).
+ */
+int orc_lin_run_ref(uint16_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {
+    if (L % 32) return -1;
+    const int nw = L / 32, W = 32;
+    LTile X;
+    X.T = X.U = X.V = W; X.nct = W / 4; X.ncu = W / 4;
+    int nsite = W * W * W;
+    X.lattice = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 4));
+    X.slab = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 8));
+    gpu_trans(X.tr);
+    for (int i = 0; i < n_time; i++) {
+        int ot = origins[3 * i], ou = origins[3 * i + 1], ov = origins[3 * i + 2];
+        for (int wid = 0; wid < nw * nw * nw; wid++) {
+            int wt = wid % nw, wu = (wid / nw) % nw, wv = wid / (nw * nw);
+            memset(X.lattice, 0, sizeof(uint32_t) * (nsite / 4));
+            memset(X.slab, 0, sizeof(uint32_t) * (nsite / 8));
+            for (int v = 0; v < W; v++)
+                for (int u = 0; u < W; u++)
+                    for (int t = 0; t < W; t++) {
+                        size_t g = (size_t)((ot + wt * W + t) % L) + (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L));
+                        uint32_t w = lat[g], sidx = lidx(&X, t, u, v);
+                        X.lattice[sidx / 4] |= (w & 0xff) << (8 * (sidx % 4));
+                        X.slab[sidx / 8] |= ((w >> 8) & 1) << (16 | (sidx & 7));
+                        X.slab[sidx / 8] |= ((w >> 9) & 3) << ((sidx & 7) * 2);
+                    }
+            uint32_t* st = seeds + (size_t)wid * 512 * 8;
+            for (int it = 0; it < n_step; it++) {
+                for (int s = 0; s < 2; s++) {
+                    uint32_t off = 0;
+                    for (int lid = 0; lid < 512; lid++) off = lfsr113_dev(st + 8 * lid + 4) & 0x88f;  /* identical per lane */
+                    int ot2 = (int)(off & 1) + 2 * (int)((off >> 3) & 1);
+                    int ou2 = (int)((off >> 1) & 1) + 2 * (int)((off >> 7) & 1);
+                    int ov2 = (int)((off >> 2) & 1) + 2 * (int)((off >> 11) & 1);
+                    for (int lid = 0; lid < 512; lid++) {
+                        uint32_t site = lidx(&X, 4 * (lid & 7) + ot2, 4 * ((lid >> 3) & 7) + ou2, 4 * ((lid >> 6) & 7) + ov2);
+                        if (s == 0) diffuse_sl(&X, site);
+                        else {
+                            LDraw rd = {1, 0, st + 8 * lid};
+                            trans_move(&X, site, &rd);
+                        }
+                    }
+                }
+            }
+            for (int v = 0; v < W; v++)
+                for (int u = 0; u < W; u++)
+                    for (int t = 0; t < W; t++) {
+                        size_t g = (size_t)((ot + wt * W + t) % L) + (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L));
+                        uint32_t sidx = lidx(&X, t, u, v);
+                        uint32_t w = (X.lattice[sidx / 4] >> (8 * (sidx % 4))) & 0xff;
+                        w |= ((X.slab[sidx / 8] >> (16 | (sidx & 7))) & 1) << 8;
+                        w |= ((X.slab[sidx / 8] >> ((sidx & 7) * 2)) & 3) << 9;
+                        lat[g] = (uint16_t)w;
+                    }
+        }
+    }
     free(X.lattice);
     free(X.slab);
     return 0;

@@ -10,7 +10,7 @@ GPULIB_RING = ["GPULibInit", "GPULibLoadBuffers", "GPULibRun", "GPULibEqRun", "G
                "GPULibRelease"]
 GPULIB_LIN = ["GPULibInit", "GPULibLoadBuffers", "GPULibRun", "GPULibRelease", "GPULatticeToCPU"]
 HOST = ["new", "from_file", "free", "info", "get_lattice", "set_lattice", "write_tuv", "polymer", "load_shift",
-        "origins", "seed"]
+        "origins", "seed", "seeds"]
 
 
 def test_libelp_exports_every_declared_symbol():
