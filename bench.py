@@ -29,8 +29,8 @@ sys.path.insert(0, PKG)
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(L, N, density, cores, warm, timed):
-    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
+def cpu_baseline(L, N, density, cores, warm, timed, ring=True):
+    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring" if ring else "efpol_port_lin")
     if not os.path.exists(exe):
         return None
     procs = [subprocess.Popen([exe, "bench", str(L), str(N), str(density), str(1000 + i), str(timed), str(warm)],
@@ -45,7 +45,8 @@ def cpu_baseline(L, N, density, cores, warm, timed):
         "unit": "attempts/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"efpol restatement, ring L={L} N={N} density {density}, {cores} independent processes "
+        "sample": (f"efpol restatement, {'ring' if ring else 'linear'} L={L} N={N} density {density}, "
+                   f"{cores} independent processes "
                    f"(seeds 1000..{1000 + cores - 1}), {warm} untimed + {timed} timed time units from efpol's "
                    f"two-site-loop start; DoMCStep region only; 1 attempt = 1 monomer move attempt"),
     }
@@ -57,7 +58,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=128)
-    ap.add_argument("--N", type=int, default=1000)
+    ap.add_argument("--N", type=int, default=None, help="monomers per chain (ring 1000, linear 500)")
+    ap.add_argument("--chain", default="ring", choices=["ring", "linear"])
     ap.add_argument("--density", type=float, default=1.2)
     ap.add_argument("--tile", default="16")
     ap.add_argument("--sublattice", default="dense", choices=["cubic", "dense"])
@@ -84,13 +86,18 @@ def main():
 
     tile = tuple(int(x) for x in args.tile.split("x")) if "x" in args.tile else (int(args.tile),) * 3
     dense = args.sublattice == "dense"
-    n_iter = args.n_iter or (13 if dense else 51)
-    host = elp.RingHost(N=args.N, L=args.L, density=args.density, seed=0)
-    eng = elp.Engine("ring", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC, n_iter)
+    ring = args.chain == "ring"
+    substeps = 4 if ring else 2
+    n_iter = args.n_iter or ((13 if dense else 51) if ring else (16 if dense else 64))
+    N = args.N or (1000 if ring else 500)
+    host = (elp.RingHost if ring else elp.LinHost)(N=N, L=args.L, density=args.density, seed=0)
+    eng = elp.Engine("ring" if ring else "linear", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC,
+                     n_iter)
     eng.upload(host.lattice())
     host.load_shift(tile)
     nsite = host.L[0] * host.L[1] * host.L[2]
-    attempts_per_step = n_iter * 4 * (nsite // 64) * (4 if dense else 1)
+    site_bytes = 1 if ring else 2
+    attempts_per_step = n_iter * substeps * (nsite // 64) * (4 if dense else 1)
 
     tau = 0
     if args.warmup:
@@ -110,14 +117,14 @@ def main():
 
     value = attempts_per_step * args.steps / wall
     launch_s = kernel_ms * 1e-3 / args.steps
-    bytes_per_launch = 2 * nsite
+    bytes_per_launch = 2 * site_bytes * nsite
     achieved = bytes_per_launch / launch_s / 1e9
     info = eng.launch_info()
 
     copy_ms = elp.bench_copy(bytes_per_launch // 2, 200)
     copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
     cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
-    cpu = None if args.no_cpu else cpu_baseline(args.L, args.N, args.density, cores, warm=300, timed=100)
+    cpu = None if args.no_cpu else cpu_baseline(args.L, N, args.density, cores, warm=300, timed=100, ring=ring)
 
     rec = {
         "metric": "MC move attempts/s, %HBM roofline; equil. wall-time learned vs plain",
@@ -130,11 +137,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8",
+        "dtype": "u8" if ring else "u16",
         "data": "synthetic",
         "config": {
-            "workload": f"ring melt L={host.L[0]} N={host.N} density {args.density} ({host.n_pol} rings), "
-                        f"one time unit per step",
+            "workload": f"{'ring' if ring else 'linear-chain'} melt L={host.L[0]} N={host.N} density {args.density} "
+                        f"({host.n_pol} chains), one time unit per step",
             "tile": list(tile),
             "sublattice": args.sublattice,
             "iterations_per_time_unit": n_iter,
@@ -150,7 +157,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "ring_time_unit",
+            "kernel": "ring_time_unit2" if ring else "lin_time_unit2",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": launch_s * 1e6,
             "copy_kernel_same_bytes_GBs": copy_gbs,

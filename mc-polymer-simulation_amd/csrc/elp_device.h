@@ -52,4 +52,45 @@ __device__ __forceinline__ bool bond_add_valid(uint32_t a, uint32_t b, uint32_t&
     return c != 0x3u && c != 0xcu;
 }
 
+// First bond of decomposition k of bond `next` (the SetGPUTrans table,
+// gpupol2/gpupol_init.c:552-584, as a 128-bit constant: nibble 2*next+k).
+constexpr uint64_t kTransLo = 0x2172716500767500ull;   // table words 0,1
+constexpr uint64_t kTransHi = 0x0064540021b2b1a9ull;   // table words 2,3
+__device__ __forceinline__ uint32_t trans_first(uint32_t next, uint32_t k) {
+    const uint32_t e = 2u * next + k;
+    const uint64_t w = (e & 16u) ? kTransHi : kTransLo;
+    return (uint32_t)(w >> (4u * (e & 15u))) & 0xfu;
+}
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+// Packed tile coordinates P = t | u<<SU | v<<SV with one guard bit per field
+// (tile edges are powers of two): a neighbour is P + delta(bond), "outside
+// the tile" is (P & GUARD) != 0, and the LDS index of a site is either P
+// itself (PAD=1, sparse: 4x the tile in bytes) or the compact t-fastest
+// index rebuilt from P (PAD=0).  addr() masks P into the tile so that a
+// speculative read of a rejected neighbour stays inside the array.
+template <int TT, int TU, int TV, int PAD>
+struct PTile {
+    static constexpr int LT = ilog2c(TT), LU = ilog2c(TU), LV = ilog2c(TV);
+    static_assert((1 << LT) == TT && (1 << LU) == TU && (1 << LV) == TV, "tile edges must be powers of two");
+    static constexpr uint32_t SU = LT + 1, SV = LT + LU + 2;
+    static constexpr uint32_t ONES = 1u | (1u << SU) | (1u << SV);
+    static constexpr uint32_t GUARD = (uint32_t)TT | ((uint32_t)TU << SU) | (~0u << (SV + LV));
+    static constexpr uint32_t MASK = (uint32_t)(TT - 1) | ((uint32_t)(TU - 1) << SU) | ((uint32_t)(TV - 1) << SV);
+    static constexpr int NSITE = PAD ? (1 << (SV + LV)) : TT * TU * TV;
+    __device__ static __forceinline__ uint32_t pack(uint32_t t, uint32_t u, uint32_t v) { return t | (u << SU) | (v << SV); }
+    __device__ static __forceinline__ uint32_t delta(uint32_t b) {
+        const uint32_t d = (b & 1u) | ((b & 2u) << (SU - 1)) | ((b & 4u) << (SV - 2));
+        return (b & 8u) ? d - ONES : d;
+    }
+    __device__ static __forceinline__ bool out(uint32_t P) { return (P & GUARD) != 0u; }
+    __device__ static __forceinline__ uint32_t addr(uint32_t P) {
+        P &= MASK;
+        if (PAD) return P;
+        return (P & (uint32_t)(TT - 1)) | ((P >> 1) & ((uint32_t)(TU - 1) << LT)) |
+               ((P >> 2) & ((uint32_t)(TV - 1) << (LT + LU)));
+    }
+};
+
 }  // namespace elp

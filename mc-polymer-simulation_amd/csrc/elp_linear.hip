@@ -245,22 +245,225 @@ __global__ __launch_bounds__(THREADS) void lin_time_unit(uint16_t* __restrict__ 
     }
 }
 
-#define ELP_LIN_CASE(TT, TU, TV, DENSE, TH)                                                                 \
-    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                               \
-        if (probe) return 0;                                                                                 \
-        const size_t lds = sizeof(uint16_t) * TT * TU * TV;                                                  \
-        if (lds > 65536) {                                                                                   \
-            hipError_t e = hipFuncSetAttribute((const void*)lin_time_unit<TT, TU, TV, DENSE, TH>,             \
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);        \
-            if (e != hipSuccess) return (int)e;                                                              \
-        }                                                                                                    \
-        hipLaunchKernelGGL((lin_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), lds, stream, glat, \
-                           p);                                                                               \
-        return (int)hipGetLastError();                                                                       \
+// =====================================================================
+// Second formulation (default): packed guarded coordinates (PTile, see
+// elp_device.h), all neighbour words a move may need read right after the
+// site word, then one accept/reject decision per case.  Same per-site
+// results as the functions above.
+// =====================================================================
+template <class G>
+__device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
+    const uint32_t a = G::addr(P);
+    uint32_t w = lat[a];
+    const uint32_t next = w & 0xfu;
+    const uint32_t Pn = P + G::delta(next);
+    const uint32_t na = G::addr(Pn);
+    uint32_t x = lat[na];
+    if (!next || G::out(Pn)) return;
+    if ((w & kSL) && !(x & kSL)) {
+        const uint32_t l0 = (w >> 9) & 1u, l1 = (w >> 10) & 1u;
+        w = (w & ~(kL0 | kL1 | kSL)) | (l1 << 9);
+        x |= (l0 << 10) | kSL;
+    } else if (!(w & kSL) && (x & kSL)) {
+        const uint32_t q = (x >> 10) & 1u;
+        x &= ~(kL1 | kSL);
+        w = (w & ~kL0) | (((w >> 9) & 1u) << 10) | (q << 9) | kSL;
+    } else {
+        return;
+    }
+    lat[a] = (uint16_t)w;
+    lat[na] = (uint16_t)x;
+}
+
+template <class G>
+__device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
+    const uint32_t a = G::addr(P);
+    uint32_t w = lat[a];
+    const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
+    const bool sl = (w & kSL) != 0u;
+    // stored-length cases: split the chosen bond, or grow the end
+    const uint32_t pbS = (r >> 2) & 1u;
+    const uint32_t bondS = pbS ? prev : next;
+    const uint32_t nb = (r >> 3) & 0xfu;
+    uint32_t nPrev = trans_first(bondS, r & 1u);
+    uint32_t nNext = bond_add((~nPrev) & 0xfu, bondS);
+    if (!(r & 2u)) { const uint32_t x = nPrev; nPrev = nNext; nNext = x; }
+    const uint32_t B = P + G::delta(pbS ? (bondS ^ 0xfu) : bondS);   // bond partner
+    const uint32_t N = (pbS ? B : P) + G::delta(nPrev);               // inserted site
+    const uint32_t Gr = P + G::delta(nb);                             // growth site
+    // cases without stored length: retract an end, or fold an interior monomer
+    const bool end = (next == 0u) || (prev == 0u);
+    const uint32_t bondE = (next == 0u) ? (prev ^ 0xfu) : next;
+    const uint32_t E = P + G::delta(bondE);
+    const uint32_t Pp = P + G::delta((~prev) & 0xfu), Pn = P + G::delta(next);
+    uint32_t X1, X2;
+    if (sl) { X1 = bondS ? N : Gr; X2 = bondS ? B : Gr; }
+    else { X1 = end ? E : Pp; X2 = end ? E : Pn; }
+    const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
+    uint32_t x1 = lat[a1];
+    uint32_t x2 = lat[a2];
+    if (!(w & 0xffu)) return;
+    if (sl) {
+        if (bondS == 0u) {   // grow the chain end out of the stored length
+            const uint32_t pb = pbS ^ 1u;
+            if (nb == 0u || nb == 3u || nb == 0xcu || nb == 0xfu || G::out(Gr) || (x1 & 0xffu)) return;
+            const uint32_t ins = pb ? nb : (nb ^ 0xfu);
+            const uint32_t lab = (w >> 9) & 3u;
+            w = (w & ~(kL0 | kL1 | kSL)) | (((lab >> pb) & 1u) << 9) | (ins << (4 * (pb ^ 1u)));
+            x1 |= (((lab >> (pb ^ 1u)) & 1u) << 9) | (ins << (4 * pb));
+            lat[a] = (uint16_t)w;
+            lat[a1] = (uint16_t)x1;
+            return;
+        }
+        if (G::out(B) || G::out(N) || (x1 & 0xffu)) return;   // split the bond through an empty site
+        const uint32_t lab = (w >> (9 + pbS)) & 1u;
+        x1 |= nNext | (nPrev << 4) | (lab << 9);
+        if (pbS) {
+            w = (w & ~(0xf0u | kSL | kL1)) | (nNext << 4);
+            x2 = (x2 & ~0xfu) | nPrev;
+        } else {
+            w = (w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9);
+            x2 = (x2 & ~0xf0u) | (nNext << 4);
+        }
+        lat[a] = (uint16_t)w;
+        lat[a1] = (uint16_t)x1;
+        lat[a2] = (uint16_t)x2;
+        return;
+    }
+    if ((r >> 4) & 1u) return;
+    if (end) {   // retract a chain end onto its neighbour as stored length
+        if (((r >> 5) & 7u) || G::out(E) || (x1 & kSL)) return;
+        const uint32_t pb = (next == 0u) ? 1u : 0u;
+        const uint32_t nl = (((w >> 9) & 1u) << (pb ^ 1u)) | (((x1 >> 9) & 1u) << pb);
+        w &= ~(kL0 | (0xfu << (4 * pb)));
+        x1 = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pb ^ 1u))))) | (nl << 9) | kSL;
+        lat[a] = (uint16_t)w;
+        lat[a1] = (uint16_t)x1;
+        return;
+    }
+    // fold the monomer onto its prev (r bit 0) or next neighbour as stored length
+    uint32_t bond;
+    if (G::out(Pp) || G::out(Pn) || !bond_add_valid(next, prev, bond)) return;
+    const uint32_t pb = r & 1u;
+    if ((pb ? x1 : x2) & kSL) return;
+    const uint32_t lab = (w >> 9) & 1u;
+    if (pb) x1 = (((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (lab << 9));
+    else x2 |= kSL | (lab << 10);
+    x1 = (x1 & ~0xfu) | bond;
+    x2 = (x2 & ~0xf0u) | (bond << 4);
+    lat[a] = (uint16_t)(w & ~(kL0 | 0xffu));
+    lat[a1] = (uint16_t)x1;
+    lat[a2] = (uint16_t)x2;
+}
+
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__global__ __launch_bounds__(THREADS) void lin_time_unit2(uint16_t* __restrict__ glat, LinLaunch p) {
+    using G = PTile<TT, TU, TV, PAD>;
+    constexpr int NCT = TT / 4, NCU = TU / 4;
+    constexpr int NCELL = (TT / 4) * (TU / 4) * (TV / 4);
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NSLOT = NCELL * SPC;
+    constexpr int SPT = NSLOT / THREADS;
+    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+    extern __shared__ uint16_t tile[];
+
+    const int tid = threadIdx.x;
+    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i] =
+            glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+
+    uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        slot[k] = (uint32_t)(tid + k * THREADS);
+        const uint32_t cell = slot[k] / SPC, j = slot[k] % SPC;
+        base[k] = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
+        j3[k] = 3u * j;
+        j2[k] = 2u * j;
+    }
+    __syncthreads();
+
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        uint4 sw[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+        for (int j = 0; j < nj; j++) {
+            const uint32_t offw = word_of(ow, j);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+#pragma unroll
+                for (int k = 0; k < SPT; k++) {
+                    uint32_t P;
+                    if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
+                    else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
+                    if (s == 0) lin_diffuse_sl2<G>(tile, P);
+                    else lin_trans_move2<G>(tile, P, (word_of(sw[k], j) >> 8) & 0xffu);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] =
+            tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i];
+    }
+}
+
+// variant 1: lin_time_unit (cell-major LDS); 2: lin_time_unit2 compact;
+// 3: lin_time_unit2 with P-indexed LDS for 16^3 tiles (default).
+template <class K>
+static int launch_lds(K kernel, size_t lds, int nblocks, int threads, hipStream_t stream, uint16_t* glat,
+                      const LinLaunch& p) {
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(threads), lds, stream, glat, p);
+    return (int)hipGetLastError();
+}
+
+#define ELP_LIN_CASE(TT, TU, TV, DENSE, TH)                                                                   \
+    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                                 \
+        if (probe) return 0;                                                                                   \
+        if (variant == 1)                                                                                      \
+            return launch_lds(lin_time_unit<TT, TU, TV, DENSE, TH>, sizeof(uint16_t) * TT * TU * TV, nblocks, TH, \
+                              stream, glat, p);                                                                \
+        if constexpr (TT == 16 && TU == 16 && TV == 16)                                                        \
+            if (variant == 3)                                                                                  \
+                return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, 1>,                                    \
+                                  sizeof(uint16_t) * PTile<TT, TU, TV, 1>::NSITE, nblocks, TH, stream, glat, p); \
+        return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, 0>, sizeof(uint16_t) * TT * TU * TV, nblocks, TH, \
+                          stream, glat, p);                                                                    \
     }
 
 int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                uint16_t* glat, const LinLaunch& p) {
+    static const int variant = [] {
+        const char* env = getenv("ELP_LIN_VARIANT");
+        return env ? atoi(env) : 3;
+    }();
     ELP_LIN_CASE(16, 16, 16, false, 64)
     ELP_LIN_CASE(16, 16, 16, true, 64)
     ELP_LIN_CASE(16, 16, 16, true, 128)

@@ -229,17 +229,211 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
     }
 }
 
+// =====================================================================
+// Second formulation of the same kernel (selected by default): sites are
+// addressed by packed tile coordinates P = t | u<<SU | v<<SV whose fields
+// carry one guard bit each, so
+//   * a neighbour is P + delta(bond) (bit arithmetic, no per-axis code),
+//   * "outside the tile" is one AND with the guard bits,
+//   * the LDS tile is plain t-fastest (PAD=0: index rebuilt from P in three
+//     bit-field operations) or indexed by P itself (PAD=1, 16^3 tiles only),
+//   * every LDS read a move needs is issued before the accept/reject test,
+//     so TransForw and DiffuseSL wait for two dependent LDS round trips and
+//     TransBack for three, and the trans table is a 128-bit constant in
+//     registers.  Accept/reject rules and writes are those of the functions
+//     above.
+// =====================================================================
+template <class G>
+__device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
+    const uint32_t a = G::addr(P);
+    uint32_t s = lat[a];
+    const uint32_t next = s & 0xfu;
+    const uint32_t Pn = P + G::delta(next);
+    uint32_t nb1 = trans_first(next, r & 1u);
+    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
+    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+    const uint32_t Pd = P + G::delta(nb1);
+    const uint32_t na = G::addr(Pn), da = G::addr(Pd);
+    uint32_t ns = lat[na];
+    const uint32_t ds = lat[da];
+    const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u;
+    if (!s || G::out(Pn | Pd) || !(sl | nsl) || ds) return;
+    const bool moveFirst = (sl && nsl) ? ((r >> 2) & 1u) : (sl != 0);
+    uint32_t d = nb2;
+    if (moveFirst) {
+        s = nb1 | ((label >> 1) & 0x10u);
+        d |= label & 0x10u;
+    } else {
+        s = nb1 | label | sl;
+        d |= (ns & 0x20u) >> 1;
+        ns &= 0x1fu;
+    }
+    lat[a] = (uint8_t)s;
+    lat[da] = (uint8_t)d;
+    if (!moveFirst) lat[na] = (uint8_t)ns;
+}
+
+template <class G>
+__device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r) {
+    const uint32_t a = G::addr(P);
+    uint32_t s = lat[a];
+    const uint32_t next = s & 0xfu;
+    const uint32_t Ps = P + G::delta(next);
+    const uint32_t sa = G::addr(Ps);
+    const uint32_t ss = lat[sa];
+    const uint32_t srcNext = ss & 0xfu;
+    const bool combine = !((next | srcNext) != 0xfu && (next & srcNext));
+    const uint32_t newNext = bond_add(next, srcNext);
+    const uint32_t Pw = P + G::delta(newNext);
+    const uint32_t wa = G::addr(Pw);
+    uint32_t ws = lat[wa];
+    const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
+    if (!s || G::out(Ps | Pw) || (ss & 0x40u) || !combine || newNext == 0x3u || newNext == 0xcu || (sl && wsl)) return;
+    const bool moveFirst = (!sl && !wsl) ? (r & 1u) : (sl == 0);
+    const uint32_t srcLabel = ss & 0x30u;
+    if (moveFirst) {
+        s = newNext | (label << 1) | srcLabel | 0x40u;
+    } else {
+        s = newNext | label | sl;
+        ws = (ws & 0x3fu) | (srcLabel << 1) | 0x40u;
+    }
+    lat[sa] = 0;
+    lat[a] = (uint8_t)s;
+    lat[wa] = (uint8_t)ws;
+}
+
+template <class G>
+__device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
+    const uint32_t a = G::addr(P);
+    uint32_t s = lat[a];
+    const uint32_t next = s & 0xfu;
+    const uint32_t Pn = P + G::delta(next);
+    const uint32_t na = G::addr(Pn);
+    uint32_t ns = lat[na];
+    const uint32_t label = s & 0x30u, sl = s & 0x40u;
+    if (!s || G::out(Pn) || ((ns & 0x40u) + sl) != 0x40u) return;
+    if (sl) {
+        ns = ns | ((label & 0x10u) << 1) | 0x40u;
+        s = next | ((label >> 1) & 0x10u);
+    } else {
+        s = next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u;
+        ns &= 0x1fu;
+    }
+    lat[a] = (uint8_t)s;
+    lat[na] = (uint8_t)ns;
+}
+
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p) {
+    using G = PTile<TT, TU, TV, PAD>;
+    constexpr int NCT = TT / 4, NCU = TU / 4;
+    constexpr int NCELL = (TT / 4) * (TU / 4) * (TV / 4);
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NSLOT = NCELL * SPC;
+    constexpr int SPT = NSLOT / THREADS;
+    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+    __shared__ uint8_t tile[G::NSITE];
+
+    const int tid = threadIdx.x;
+    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i] =
+            glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+
+    uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        slot[k] = (uint32_t)(tid + k * THREADS);
+        const uint32_t cell = slot[k] / SPC, j = slot[k] % SPC;
+        base[k] = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
+        j3[k] = 3u * j;
+        j2[k] = 2u * j;
+    }
+    __syncthreads();
+
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        uint4 sw[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+        for (int j = 0; j < nj; j++) {
+            const uint32_t offw = word_of(ow, j);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+#pragma unroll
+                for (int k = 0; k < SPT; k++) {
+                    uint32_t P;
+                    if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
+                    else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
+                    const uint32_t r = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
+                    if (s == 1) diffuse_sl2<G>(tile, P);
+                    else if (s == 3) trans_back2<G>(tile, P, r);
+                    else trans_forw2<G>(tile, P, r);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] =
+            tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i];
+    }
+}
+
+// P-indexed LDS only for 16^3 tiles (16 KiB); returns whether it launched
+template <int TT, int TU, int TV, bool DENSE, int TH>
+static bool launch_padded(int variant, int nblocks, hipStream_t stream, uint8_t* glat, const RingLaunch& p) {
+    if constexpr (TT == 16 && TU == 16 && TV == 16) {
+        if (variant == 3) {
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 1>), dim3(nblocks), dim3(TH), 0, stream, glat, p);
+            return true;
+        }
+    }
+    return false;
+}
+
 // ---- dispatch over the compiled schedules ----
-#define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                   \
-    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                  \
-        if (probe) return 0;                                                                    \
-        hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, \
-                           stream, glat, p);                                                    \
-        return (int)hipGetLastError();                                                          \
+// variant 1: ring_time_unit (cell-major LDS); 2: ring_time_unit2 compact;
+// 3: ring_time_unit2 with P-indexed (padded) LDS.
+#define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                              \
+    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
+        if (probe) return 0;                                                                               \
+        if (variant == 1)                                                                                  \
+            hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, stream, \
+                               glat, p);                                                                   \
+        else if (!launch_padded<TT, TU, TV, DENSE, TH>(variant, nblocks, stream, glat, p))               \
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 0>), dim3(nblocks), dim3(TH), 0,    \
+                               stream, glat, p);                                                           \
+        return (int)hipGetLastError();                                                                     \
     }
 
 int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                 uint8_t* glat, const RingLaunch& p) {
+    // default: P-indexed LDS for 16^3 tiles (fastest measured), compact otherwise
+    static const int variant = [] {
+        const char* env = getenv("ELP_RING_VARIANT");
+        return env ? atoi(env) : 3;
+    }();
     ELP_RING_CASE(16, 16, 16, false, 64)
     ELP_RING_CASE(16, 16, 16, true, 64)
     ELP_RING_CASE(16, 16, 16, true, 128)

@@ -25,7 +25,7 @@
  * length per site, cyclic label sequence of every ring unchanged); exact
  * stationary-distribution checks on tiny systems; and, in reference mode,
  * bit-equality with the reference kernel gpupol2/ringpol3.cu compiled
- * unmodified (tests/test_ring_refkernel.py).
+ * unmodified (tests/test_refkernel_gpu.py).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -274,8 +274,10 @@ int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
  * xorshift128 states rngp/rngl taken from and written back to the seed table
  * of gpupol2/gpupol_init.c:22-33 (uint4 pair per lane: rngp, rngl), one rngl
  * draw per iteration giving the four 6-bit cell offsets (:279-291).
- * origins[3i..3i+2] is the tile-grid origin (accumulated host shift) of time
- * unit i.  Judge: the reference kernel compiled unmodified
+ * origins[3i..3i+2] is the accumulated host shift of time unit i modulo L:
+ * work-group w (and its seed streams) owns the tile starting at
+ * origin + 32*(w%nw, (w/nw)%nw, w/nw^2) (gpupol2/oclpol.c:18-47), so the
+ * shift is needed mod L, not mod 32.  Judge: the reference kernel compiled unmodified
  * (oracle/refkernel_ring.hip).
  */
 int orc_ring_run_ref(uint8_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {

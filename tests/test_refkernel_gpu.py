@@ -55,7 +55,9 @@ def test_ringpol3_reference_kernel_matches_oracle(tmp_path, L, N, seed, ntime):
     lib.orc_ring_run_ref.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     want = lat0.copy()
     want_seeds = seeds0.copy()
-    o = np.ascontiguousarray(org[:ntime], dtype=np.int32)
+    # work-group w owns the tile at (accumulated shift mod L) + 32*w: the
+    # binding of seed streams to tiles needs the shift mod L, not mod 32
+    o = np.ascontiguousarray(np.cumsum(d, axis=0)[:ntime] % L, dtype=np.int32)
     assert lib.orc_ring_run_ref(want.ctypes.data, L, want_seeds.ctypes.data, 51, ntime, o.ctypes.data) == 0
     assert np.count_nonzero(want != lat0) > 1000
     assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
