@@ -52,6 +52,21 @@ def cpu_baseline(L, N, density, cores, warm, timed, ring=True):
     }
 
 
+def measured_traffic(kernel, chain, L, N, tile, sublattice):
+    """HBM bytes per launch recorded from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this exact
+    workload (profiles/hbm_traffic.txt), or None when the workload has not been profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    key = [kernel, chain, str(L), str(N), "x".join(str(t) for t in tile), sublattice]
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 7 and not line.startswith("#") and w[:6] == key:
+                return int(w[6])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +138,8 @@ def main():
 
     copy_ms = elp.bench_copy(bytes_per_launch // 2, 200)
     copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
+    kernel = "ring_time_unit2" if ring else "lin_time_unit2"
+    traffic = measured_traffic(kernel, args.chain, host.L[0], host.N, tile, args.sublattice)
     cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     cpu = None if args.no_cpu else cpu_baseline(args.L, N, args.density, cores, warm=300, timed=100, ring=ring)
 
@@ -156,8 +173,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "ring_time_unit2" if ring else "lin_time_unit2",
+            "traffic": traffic,
+            "traffic_source": "profiles/hbm_traffic.txt (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
+                              if traffic else None,
+            "kernel": kernel,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": launch_s * 1e6,
             "copy_kernel_same_bytes_GBs": copy_gbs,
