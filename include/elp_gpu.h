@@ -100,6 +100,48 @@ int elp_last_run_ms(elp_engine* e, float* ms);
 /* Kernel launch geometry chosen for this engine (for reports). */
 int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_unit, int* lds_bytes);
 
+/* Threads per workgroup (each thread runs tile slots / threads slots of a
+   substep); ELP_E_ARG if that launch shape is not compiled in.  Results do
+   not depend on it. */
+int elp_set_threads(elp_engine* e, int threads);
+
+/*
+ * Batched replicas (no reference counterpart: the reference runs one
+ * independent system per process or per device, gpupol2/gpupol_init.c:14-17,
+ * gpupol2/gpupol2.c:52-55).  n_rep independent lattices of any boxes
+ * boxes[3*r+0..2] (each divisible by the common tile tt x tu x tv) share one
+ * device buffer; one launch per time unit runs every tile of every replica,
+ * and elp_batch_run can capture units_per_graph consecutive launches in a
+ * HIP graph that it replays.  Replica r evolves exactly as an elp_engine of
+ * the same box and schedule run with the same origins, tau and key.
+ */
+typedef struct elp_batch elp_batch;     /* opaque */
+
+int elp_ring_batch_create(elp_batch** out, int device, int n_rep, const int32_t* boxes,
+                          int tt, int tu, int tv, int sublattice, int n_iter);
+int elp_lin_batch_create(elp_batch** out, int device, int n_rep, const int32_t* boxes,
+                         int tt, int tu, int tv, int sublattice, int n_iter);
+int elp_batch_destroy(elp_batch* b);
+
+/* Copy replica rep's lattice host->device / device->host (synchronous). */
+int elp_batch_upload(elp_batch* b, int rep, const void* lattice, uint64_t nbytes);
+int elp_batch_download(elp_batch* b, int rep, void* lattice, uint64_t nbytes);
+
+/*
+ * Run n_time time units of every replica.  origins[3*(i*n_rep + r) + 0..2] =
+ * tile-grid origin of replica r in time unit i; replica r uses Philox counter
+ * tau0[r] + i and key (keys[2r], keys[2r+1]).  units_per_graph > 0 captures
+ * that many launches in a HIP graph (kept for later calls) and replays it
+ * n_time / units_per_graph times; the remainder is launched directly.
+ * 0 launches every time unit directly.  Asynchronous on the batch's stream.
+ */
+int elp_batch_run(elp_batch* b, int n_time, const int32_t* origins, const uint64_t* tau0,
+                  const uint32_t* keys, int units_per_graph);
+int elp_batch_sync(elp_batch* b);
+int elp_batch_last_run_ms(elp_batch* b, float* ms);
+int elp_batch_info(elp_batch* b, int* threads_per_block, int* blocks_per_time_unit, uint64_t* sites);
+int elp_batch_set_threads(elp_batch* b, int threads);
+
 /* Diagnostic (no reference counterpart): device-to-device copy of nbytes
    (multiple of 16) with a plain streaming kernel, reps times; *ms = average
    time per copy.  Used to calibrate HBM byte counters and to measure the

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <vector>
 #include "../../include/elp_gpu.h"
 #include "elp_kernels.h"
 
@@ -235,6 +236,324 @@ int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_
     if (threads_per_block) *threads_per_block = e->threads;
     if (blocks_per_time_unit) *blocks_per_time_unit = (e->LT / e->tt) * (e->LU / e->tu) * (e->LV / e->tv);
     if (lds_bytes) *lds_bytes = (int)(e->tt * e->tu * e->tv * (e->kind == 0 ? 1 : 2));
+    return ELP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched replicas
+// ---------------------------------------------------------------------------
+
+__global__ void elp_set_cursor(int* c, int v) { *c = v; }
+__global__ void elp_advance_cursor(int* c, int v) { *c += v; }
+
+struct elp_batch {
+    int kind;
+    int device;
+    int n_rep;
+    int tt, tu, tv;
+    int sublattice;
+    int n_iter;
+    int threads;
+    size_t bytes_per_site;
+    std::vector<int32_t> box;              // 3 per replica
+    std::vector<elp::BatchRep> reps;
+    uint64_t n_sites;                      // sites of the buffer (incl. alignment gaps)
+    uint32_t n_tiles;
+    void* d_lat = nullptr;
+    elp::BatchRep* d_reps = nullptr;
+    uint32_t* d_rep_of_tile = nullptr;
+    elp::BatchStep* d_steps = nullptr;
+    size_t steps_rows = 0;                 // capacity of d_steps in time units
+    int* d_cursor = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<char> uploaded;
+    bool timed = false;
+    std::vector<elp::BatchStep> h_steps;   // staging of the last run's schedule
+    hipGraphExec_t gexec = nullptr;        // cached graph of g_units launches + cursor advance
+    int g_units = 0;
+    const void* g_steps = nullptr;
+};
+
+static int batch_launch_one(elp_batch* b, bool probe, int k) {
+    elp::BatchLaunch bl{b->d_reps, b->d_rep_of_tile, b->d_steps, b->d_cursor, k, b->n_rep, b->n_iter};
+    const bool dense = b->sublattice == ELP_SUBLAT_DENSE;
+    if (b->kind == 0)
+        return elp::ring_batch_launch(b->tt, b->tu, b->tv, dense, b->threads, probe, (int)b->n_tiles, b->stream,
+                                      (uint8_t*)b->d_lat, bl);
+    return elp::lin_batch_launch(b->tt, b->tu, b->tv, dense, b->threads, probe, (int)b->n_tiles, b->stream,
+                                 (uint16_t*)b->d_lat, bl);
+}
+
+static void batch_free(elp_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    (void)hipFree(b->d_lat);
+    (void)hipFree(b->d_reps);
+    (void)hipFree(b->d_rep_of_tile);
+    (void)hipFree(b->d_steps);
+    (void)hipFree(b->d_cursor);
+    delete b;
+}
+
+static int batch_create(elp_batch** out, int kind, int device, int n_rep, const int32_t* boxes, int tt, int tu, int tv,
+                        int sublattice, int n_iter) {
+    if (!out) return fail(ELP_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_rep <= 0 || n_rep > 65535 || !boxes) return fail(ELP_E_ARG, "n_rep %d out of [1, 65535] or boxes NULL", n_rep);
+    if (sublattice != ELP_SUBLAT_CUBIC && sublattice != ELP_SUBLAT_DENSE) return fail(ELP_E_ARG, "bad sublattice %d", sublattice);
+    if (n_iter < 0) return fail(ELP_E_ARG, "n_iter < 0");
+    elp_batch* b = new elp_batch();
+    b->kind = kind; b->device = device; b->n_rep = n_rep;
+    b->tt = tt; b->tu = tu; b->tv = tv;
+    b->sublattice = sublattice; b->n_iter = n_iter;
+    b->threads = default_threads(kind, tt, tu, tv, sublattice);
+    b->bytes_per_site = kind == 0 ? 1 : 2;
+    b->box.assign(boxes, boxes + 3 * n_rep);
+    uint64_t site = 0, tile = 0;
+    for (int r = 0; r < n_rep; r++) {
+        const int LT = boxes[3 * r], LU = boxes[3 * r + 1], LV = boxes[3 * r + 2];
+        if (LT <= 0 || LU <= 0 || LV <= 0 || LT % tt || LU % tu || LV % tv) {
+            delete b;
+            return fail(ELP_E_ARG, "replica %d: box %dx%dx%d not divisible by tile %dx%dx%d", r, LT, LU, LV, tt, tu, tv);
+        }
+        elp::BatchRep rep{};
+        rep.site_off = site;
+        rep.LT = LT; rep.LU = LU; rep.LV = LV;
+        rep.ntile_t = LT / tt; rep.ntile_u = LU / tu;
+        rep.tile0 = (uint32_t)tile;
+        b->reps.push_back(rep);
+        site += ((uint64_t)LT * LU * LV + 255) & ~(uint64_t)255;    // replicas start on 256-site boundaries
+        tile += (uint64_t)(LT / tt) * (LU / tu) * (LV / tv);
+    }
+    if (tile >= (1ull << 31)) { delete b; return fail(ELP_E_ARG, "too many tiles (%llu)", (unsigned long long)tile); }
+    b->n_sites = site;
+    b->n_tiles = (uint32_t)tile;
+    b->uploaded.assign(n_rep, 0);
+    if (batch_launch_one(b, true, 0) != 0) {
+        int th = b->threads;
+        delete b;
+        return fail(ELP_E_ARG, "batched schedule %s tile %dx%dx%d sublattice %d threads %d is not compiled in",
+                    kind == 0 ? "ring" : "linear", tt, tu, tv, sublattice, th);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { delete b; return fail(ELP_E_NODEV, "no HIP device"); }
+    if (device < 0 || device >= ndev) { delete b; return fail(ELP_E_ARG, "device %d out of range (%d devices)", device, ndev); }
+    hipError_t err;
+    std::vector<uint32_t> rep_of_tile(b->n_tiles);
+    for (int r = 0; r < n_rep; r++) {
+        const uint32_t n = (uint32_t)((b->box[3 * r] / tt) * (b->box[3 * r + 1] / tu) * (b->box[3 * r + 2] / tv));
+        for (uint32_t i = 0; i < n; i++) rep_of_tile[b->reps[r].tile0 + i] = (uint32_t)r;
+    }
+#define BCHK(call, what)                                       \
+    if ((err = (call)) != hipSuccess) {                        \
+        batch_free(b);                                         \
+        return hip_fail(err, what);                            \
+    }
+    BCHK(hipSetDevice(device), "hipSetDevice");
+    BCHK(hipMalloc(&b->d_lat, b->n_sites * b->bytes_per_site), "hipMalloc batch lattice");
+    BCHK(hipMalloc(&b->d_reps, sizeof(elp::BatchRep) * n_rep), "hipMalloc replica table");
+    BCHK(hipMalloc(&b->d_rep_of_tile, sizeof(uint32_t) * b->n_tiles), "hipMalloc tile map");
+    BCHK(hipMalloc(&b->d_cursor, sizeof(int)), "hipMalloc cursor");
+    BCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    BCHK(hipEventCreate(&b->ev0), "hipEventCreate");
+    BCHK(hipEventCreate(&b->ev1), "hipEventCreate");
+    BCHK(hipMemcpy(b->d_rep_of_tile, rep_of_tile.data(), sizeof(uint32_t) * b->n_tiles, hipMemcpyHostToDevice),
+         "hipMemcpy tile map");
+#undef BCHK
+    *out = b;
+    return ELP_OK;
+}
+
+static int batch_check_rep(elp_batch* b, int rep, const void* p, uint64_t nbytes, uint64_t* off, uint64_t* len) {
+    if (!b || !p) return fail(ELP_E_ARG, "NULL argument");
+    if (rep < 0 || rep >= b->n_rep) return fail(ELP_E_ARG, "replica %d out of range (%d replicas)", rep, b->n_rep);
+    const uint64_t n = (uint64_t)b->box[3 * rep] * b->box[3 * rep + 1] * b->box[3 * rep + 2] * b->bytes_per_site;
+    if (nbytes != n) return fail(ELP_E_ARG, "replica %d: %llu bytes, lattice is %llu", rep, (unsigned long long)nbytes,
+                                 (unsigned long long)n);
+    *off = b->reps[rep].site_off * b->bytes_per_site;
+    *len = n;
+    return ELP_OK;
+}
+
+// Graph of `units` launches (k = 0..units-1) followed by cursor += units.
+static int batch_graph(elp_batch* b, int units) {
+    if (b->gexec && b->g_units == units && b->g_steps == (const void*)b->d_steps) return ELP_OK;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < units && !rc; k++) rc = batch_launch_one(b, false, k);
+    if (!rc) {
+        hipLaunchKernelGGL(elp_advance_cursor, dim3(1), dim3(1), 0, b->stream, b->d_cursor, units);
+        rc = (int)hipGetLastError();
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return fail(ELP_E_HIP, "capture: launch failed: %s", hipGetErrorString((hipError_t)rc)); }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { b->gexec = nullptr; return hip_fail(e, "hipGraphInstantiate"); }
+    b->g_units = units;
+    b->g_steps = b->d_steps;
+    return ELP_OK;
+}
+
+extern "C" {
+
+int elp_ring_batch_create(elp_batch** out, int device, int n_rep, const int32_t* boxes, int tt, int tu, int tv,
+                          int sublattice, int n_iter) {
+    return batch_create(out, 0, device, n_rep, boxes, tt, tu, tv, sublattice, n_iter);
+}
+
+int elp_lin_batch_create(elp_batch** out, int device, int n_rep, const int32_t* boxes, int tt, int tu, int tv,
+                         int sublattice, int n_iter) {
+    return batch_create(out, 1, device, n_rep, boxes, tt, tu, tv, sublattice, n_iter);
+}
+
+int elp_batch_destroy(elp_batch* b) {
+    if (!b) return ELP_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    return ELP_OK;
+}
+
+int elp_batch_upload(elp_batch* b, int rep, const void* lattice, uint64_t nbytes) {
+    uint64_t off, len;
+    int rc = batch_check_rep(b, rep, lattice, nbytes, &off, &len);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipMemcpyAsync((char*)b->d_lat + off, lattice, len, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->uploaded[rep] = 1;
+    return ELP_OK;
+}
+
+int elp_batch_download(elp_batch* b, int rep, void* lattice, uint64_t nbytes) {
+    uint64_t off, len;
+    int rc = batch_check_rep(b, rep, lattice, nbytes, &off, &len);
+    if (rc) return rc;
+    if (!b->uploaded[rep]) return fail(ELP_E_STATE, "download of replica %d before upload", rep);
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipMemcpyAsync(lattice, (char*)b->d_lat + off, len, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return ELP_OK;
+}
+
+int elp_batch_run(elp_batch* b, int n_time, const int32_t* origins, const uint64_t* tau0, const uint32_t* keys,
+                  int units_per_graph) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    if (n_time < 0 || (n_time > 0 && (!origins || !tau0 || !keys))) return fail(ELP_E_ARG, "bad n_time/origins/tau0/keys");
+    if (units_per_graph < 0) return fail(ELP_E_ARG, "units_per_graph < 0");
+    for (int r = 0; r < b->n_rep; r++)
+        if (!b->uploaded[r]) return fail(ELP_E_STATE, "run before upload of replica %d", r);
+    const int R = b->n_rep;
+    for (int i = 0; i < n_time; i++)
+        for (int r = 0; r < R; r++) {
+            const int32_t* o = origins + 3 * ((size_t)i * R + r);
+            if (o[0] < 0 || o[0] >= b->tt || o[1] < 0 || o[1] >= b->tu || o[2] < 0 || o[2] >= b->tv)
+                return fail(ELP_E_ARG, "origin (%d,%d,%d) of replica %d, time unit %d outside the tile", o[0], o[1], o[2], r, i);
+        }
+    HIPCHK(hipSetDevice(b->device));
+    // the previous run may still read the schedule and the replica table
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (n_time == 0) return ELP_OK;
+    if ((size_t)n_time > b->steps_rows) {
+        (void)hipFree(b->d_steps);
+        b->d_steps = nullptr;
+        b->steps_rows = 0;
+        HIPCHK(hipMalloc(&b->d_steps, sizeof(elp::BatchStep) * (size_t)n_time * R));
+        b->steps_rows = (size_t)n_time;
+    }
+    b->h_steps.resize((size_t)n_time * R);
+    for (int i = 0; i < n_time; i++)
+        for (int r = 0; r < R; r++) {
+            const int32_t* o = origins + 3 * ((size_t)i * R + r);
+            b->h_steps[(size_t)i * R + r] = elp::BatchStep{o[0], o[1], o[2], (uint32_t)(tau0[r] + (uint64_t)i)};
+        }
+    for (int r = 0; r < R; r++) {
+        b->reps[r].seed = keys[2 * r];
+        b->reps[r].stream = keys[2 * r + 1];
+    }
+    HIPCHK(hipMemcpyAsync(b->d_steps, b->h_steps.data(), sizeof(elp::BatchStep) * b->h_steps.size(),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(elp::BatchRep) * R, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(elp_set_cursor, dim3(1), dim3(1), 0, b->stream, b->d_cursor, 0);
+    HIPCHK(hipGetLastError());
+    const int G = units_per_graph;
+    const int replays = G > 0 ? n_time / G : 0;
+    if (replays > 0) {
+        int rc = batch_graph(b, G);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int i = 0; i < replays; i++) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    // remainder: rows cursor + k, cursor = replays * G after the replays
+    for (int k = 0; k < n_time - replays * G; k++) {
+        int rc = batch_launch_one(b, false, k);
+        if (rc) return fail(ELP_E_HIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+    }
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    b->timed = true;
+    return ELP_OK;
+}
+
+int elp_batch_sync(elp_batch* b) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return ELP_OK;
+}
+
+int elp_batch_last_run_ms(elp_batch* b, float* ms) {
+    if (!b || !ms) return fail(ELP_E_ARG, "NULL argument");
+    if (!b->timed) return fail(ELP_E_STATE, "no run recorded");
+    HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return ELP_OK;
+}
+
+int elp_batch_info(elp_batch* b, int* threads_per_block, int* blocks_per_time_unit, uint64_t* sites) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    if (threads_per_block) *threads_per_block = b->threads;
+    if (blocks_per_time_unit) *blocks_per_time_unit = (int)b->n_tiles;
+    if (sites) {
+        uint64_t n = 0;
+        for (int r = 0; r < b->n_rep; r++) n += (uint64_t)b->box[3 * r] * b->box[3 * r + 1] * b->box[3 * r + 2];
+        *sites = n;
+    }
+    return ELP_OK;
+}
+
+int elp_batch_set_threads(elp_batch* b, int threads) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    const int old = b->threads;
+    b->threads = threads;
+    if (threads <= 0 || batch_launch_one(b, true, 0) != 0) {
+        b->threads = old;
+        return fail(ELP_E_ARG, "batched launch with %d threads is not compiled for this schedule", threads);
+    }
+    if (threads != old && b->gexec) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    return ELP_OK;
+}
+
+int elp_set_threads(elp_engine* e, int threads) {
+    if (!e) return fail(ELP_E_ARG, "NULL engine");
+    const int old = e->threads;
+    e->threads = threads;
+    elp::RingLaunch dummy{};
+    if (threads <= 0 || launch_one(e, true, dummy) != 0) {
+        e->threads = old;
+        return fail(ELP_E_ARG, "launch with %d threads is not compiled for this schedule", threads);
+    }
     return ELP_OK;
 }
 

@@ -19,6 +19,35 @@ struct RingLaunch {
 
 using LinLaunch = RingLaunch;
 
+// One replica of a batched launch (device table, read once per workgroup).
+struct BatchRep {
+    uint64_t site_off;       // first site of the replica in the batch lattice buffer
+    int LT, LU, LV;          // box
+    int ntile_t, ntile_u;    // tiles per axis (t, u)
+    uint32_t tile0;          // first global tile of the replica
+    uint32_t seed, stream;   // Philox key
+};
+
+// Tile-grid origin and Philox time-unit counter of one replica for one time unit.
+struct BatchStep {
+    int ot, ou, ov;
+    uint32_t tau;
+};
+
+// Kernel arguments of one batched time unit: every tile of every replica.
+// The time unit read is row (*cursor + k) of steps[n_time][n_rep], so a
+// captured graph of G launches (k = 0..G-1) followed by cursor += G replays
+// consecutive time units.
+struct BatchLaunch {
+    const BatchRep* reps;
+    const uint32_t* rep_of_tile;   // global tile -> replica
+    const BatchStep* steps;
+    const int* cursor;
+    int k;
+    int n_rep;
+    int n_iter;
+};
+
 // Launch one time unit (nblocks = number of tiles).  probe=true only checks
 // that the schedule is compiled in.  Returns 0, a hipError_t, or -1 when the
 // schedule is not available.
@@ -26,5 +55,10 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
                 uint8_t* glat, const RingLaunch& p);
 int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                uint16_t* glat, const LinLaunch& p);
+// Batched time unit over all replicas (nblocks = total tiles).
+int ring_batch_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
+                      uint8_t* gbase, const BatchLaunch& b);
+int lin_batch_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
+                     uint16_t* gbase, const BatchLaunch& b);
 
 }  // namespace elp

@@ -30,6 +30,7 @@
 // so the site reads of one substep hit consecutive bytes.
 #include "elp_device.h"
 #include "elp_kernels.h"
+#include "elp_tile_unit.h"
 
 namespace elp {
 
@@ -323,81 +324,30 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     lat[na] = (uint8_t)ns;
 }
 
+// ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
+// (gpupol2/ringpol3.cu:278-292) as the move set of tile_unit.
+struct RingMoves {
+    using site_t = uint8_t;
+    static constexpr int NSUB = 4;
+    template <class G>
+    __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r) {
+        if (s == 1) diffuse_sl2<G>(lat, P);
+        else if (s == 3) trans_back2<G>(lat, P, r);
+        else trans_forw2<G>(lat, P, r);
+    }
+};
+
 template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
 __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p) {
-    using G = PTile<TT, TU, TV, PAD>;
-    constexpr int NCT = TT / 4, NCU = TU / 4;
-    constexpr int NCELL = (TT / 4) * (TU / 4) * (TV / 4);
-    constexpr int SPC = DENSE ? 4 : 1;
-    constexpr int NSLOT = NCELL * SPC;
-    constexpr int SPT = NSLOT / THREADS;
-    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
-    __shared__ uint8_t tile[G::NSITE];
+    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE];
+    tile_unit<RingMoves, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p), xcd_tile(blockIdx.x, gridDim.x));
+}
 
-    const int tid = threadIdx.x;
-    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
-    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
-    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
-    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
-    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
-
-    for (int i = tid; i < TT * TU * TV; i += THREADS) {
-        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
-        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
-        gt -= (gt >= p.LT) ? p.LT : 0;
-        gu -= (gu >= p.LU) ? p.LU : 0;
-        gv -= (gv >= p.LV) ? p.LV : 0;
-        tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i] =
-            glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
-    }
-
-    uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
-#pragma unroll
-    for (int k = 0; k < SPT; k++) {
-        slot[k] = (uint32_t)(tid + k * THREADS);
-        const uint32_t cell = slot[k] / SPC, j = slot[k] % SPC;
-        base[k] = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
-        j3[k] = 3u * j;
-        j2[k] = 2u * j;
-    }
-    __syncthreads();
-
-    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
-        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
-        uint4 sw[SPT];
-#pragma unroll
-        for (int k = 0; k < SPT; k++)
-            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
-        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
-        for (int j = 0; j < nj; j++) {
-            const uint32_t offw = word_of(ow, j);
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
-#pragma unroll
-                for (int k = 0; k < SPT; k++) {
-                    uint32_t P;
-                    if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
-                    else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                    const uint32_t r = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
-                    if (s == 1) diffuse_sl2<G>(tile, P);
-                    else if (s == 3) trans_back2<G>(tile, P, r);
-                    else trans_forw2<G>(tile, P, r);
-                }
-                __syncthreads();
-            }
-        }
-    }
-
-    for (int i = tid; i < TT * TU * TV; i += THREADS) {
-        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
-        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
-        gt -= (gt >= p.LT) ? p.LT : 0;
-        gu -= (gu >= p.LU) ? p.LU : 0;
-        gv -= (gv >= p.LV) ? p.LV : 0;
-        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] =
-            tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i];
-    }
+// One time unit of every replica of a batch (one workgroup per tile).
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__ gbase, BatchLaunch b) {
+    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE];
+    batch_unit<RingMoves, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
 }
 
 // P-indexed LDS only for 16^3 tiles (16 KiB); returns whether it launched
@@ -448,6 +398,27 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
     ELP_RING_CASE(16, 16, 32, false, 128)
     ELP_RING_CASE(16, 16, 32, true, 128)
     ELP_RING_CASE(16, 16, 32, true, 256)
+    return -1;
+}
+
+// Batched launches use the second formulation (P-indexed LDS for 16^3 tiles).
+#define ELP_RING_BATCH_CASE(TT, TU, TV, DENSE, TH)                                                         \
+    if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
+        if (probe) return 0;                                                                               \
+        constexpr int PAD = (TT == 16 && TU == 16 && TV == 16) ? 1 : 0;                                    \
+        hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD>), dim3(nblocks), dim3(TH), 0, stream, \
+                           gbase, b);                                                                      \
+        return (int)hipGetLastError();                                                                     \
+    }
+
+int ring_batch_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
+                      uint8_t* gbase, const BatchLaunch& b) {
+    ELP_RING_BATCH_CASE(16, 16, 16, false, 64)
+    ELP_RING_BATCH_CASE(16, 16, 16, true, 64)
+    ELP_RING_BATCH_CASE(16, 16, 16, true, 128)
+    ELP_RING_BATCH_CASE(16, 16, 16, true, 256)
+    ELP_RING_BATCH_CASE(32, 32, 32, false, 256)
+    ELP_RING_BATCH_CASE(32, 32, 32, true, 512)
     return -1;
 }
 

@@ -1,0 +1,127 @@
+// elp_tile_unit.h -- one tile's time unit, shared by the ring and linear
+// move kernels and by their solo and batched launches.
+//
+// A workgroup stages its tile of the periodic box from HBM into LDS (indexed
+// by packed guarded coordinates, PTile), runs n_iter iterations of the move
+// set's substeps with a workgroup barrier after each substep, and writes the
+// tile back in place.  The move set M supplies the site type, the number of
+// substeps per iteration and the move of substep s:
+//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r).
+//
+// Randomness (Philox4x32-10, key = (seed, stream)):
+//   site offsets : counter (tau, tile, 0xFFFFFFFF, iter/4), word iter%4,
+//                  byte s = coset of substep s (tile-uniform);
+//   moves        : counter (tau, tile, slot, iter/4), word iter%4, byte s.
+// The tile index is the replica-local logical tile, so a replica gives the
+// same result alone or inside a batch, whatever the launch geometry.
+#pragma once
+#include "elp_device.h"
+#include "elp_kernels.h"
+
+namespace elp {
+
+// Geometry and random key of one tile's time unit.
+struct UnitArgs {
+    int LT, LU, LV;          // replica box
+    int ntile_t, ntile_u;    // tiles per axis (t, u)
+    int ot, ou, ov;          // tile-grid origin of this time unit
+    uint32_t tau, seed, stream;
+    int n_iter;
+};
+
+__device__ __forceinline__ UnitArgs unit_args(const RingLaunch& p) {
+    return UnitArgs{p.LT, p.LU, p.LV, p.ntile_t, p.ntile_u, p.ot, p.ou, p.ov, p.tau, p.seed, p.stream, p.n_iter};
+}
+
+template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat, typename M::site_t* tile,
+                                          const UnitArgs& p, uint32_t tileId) {
+    using G = PTile<TT, TU, TV, PAD>;
+    constexpr int NCT = TT / 4, NCU = TU / 4;
+    constexpr int NCELL = (TT / 4) * (TU / 4) * (TV / 4);
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NSLOT = NCELL * SPC;
+    constexpr int SPT = NSLOT / THREADS;
+    static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+
+    const int tid = threadIdx.x;
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+
+    // HBM -> LDS (periodic wrap; origins are < one tile edge, so one subtraction suffices)
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i] =
+            glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+
+    // slot k of this thread: cell corner (+ j along t for the dense cosets)
+    uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        slot[k] = (uint32_t)(tid + k * THREADS);
+        const uint32_t cell = slot[k] / SPC, j = slot[k] % SPC;
+        base[k] = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
+        j3[k] = 3u * j;
+        j2[k] = 2u * j;
+    }
+    __syncthreads();
+
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        uint4 sw[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+        for (int j = 0; j < nj; j++) {
+            const uint32_t offw = word_of(ow, j);
+#pragma unroll
+            for (int s = 0; s < M::NSUB; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+#pragma unroll
+                for (int k = 0; k < SPT; k++) {
+                    uint32_t P;
+                    if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
+                    else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
+                    M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    // LDS -> HBM
+    for (int i = tid; i < TT * TU * TV; i += THREADS) {
+        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] =
+            tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i];
+    }
+}
+
+// Workgroup of a batched launch: global tile g (XCD-aware order over all
+// replicas' tiles) -> replica r = rep_of_tile[g], its local tile g - tile0,
+// and the origin / time unit of replica r for time unit (*cursor + k).
+template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__device__ __forceinline__ void batch_unit(typename M::site_t* __restrict__ gbase, typename M::site_t* tile,
+                                           const BatchLaunch& b) {
+    const uint32_t g = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t r = b.rep_of_tile[g];
+    const BatchRep rep = b.reps[r];
+    const BatchStep st = b.steps[(size_t)(*b.cursor + b.k) * (size_t)b.n_rep + r];
+    const UnitArgs p{rep.LT, rep.LU, rep.LV, rep.ntile_t, rep.ntile_u, st.ot, st.ou, st.ov,
+                     st.tau, rep.seed, rep.stream, b.n_iter};
+    tile_unit<M, TT, TU, TV, DENSE, THREADS, PAD>(gbase + rep.site_off, tile, p, g - rep.tile0);
+}
+
+}  // namespace elp

@@ -12,6 +12,7 @@ from .core import (  # noqa: F401
     SUBLAT_CUBIC,
     SUBLAT_DENSE,
     ElpError,
+    Batch,
     Engine,
     RingHost,
     LinHost,

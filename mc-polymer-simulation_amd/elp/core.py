@@ -49,6 +49,17 @@ def load_elp():
         lib.elp_last_run_ms.argtypes = [P, C.POINTER(C.c_float)]
         lib.elp_launch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
         lib.elp_bench_copy.argtypes = [I, U64, I, C.POINTER(C.c_float)]
+        for fn in ("elp_ring_batch_create", "elp_lin_batch_create"):
+            getattr(lib, fn).argtypes = [C.POINTER(P), I, I, P, I, I, I, I, I]
+        lib.elp_batch_destroy.argtypes = [P]
+        lib.elp_batch_upload.argtypes = [P, I, P, U64]
+        lib.elp_batch_download.argtypes = [P, I, P, U64]
+        lib.elp_batch_run.argtypes = [P, I, P, P, P, I]
+        lib.elp_batch_sync.argtypes = [P]
+        lib.elp_batch_last_run_ms.argtypes = [P, C.POINTER(C.c_float)]
+        lib.elp_batch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(U64)]
+        lib.elp_batch_set_threads.argtypes = [P, I]
+        lib.elp_set_threads.argtypes = [P, I]
         lib._elp_typed = True
     return lib
 
@@ -138,6 +149,73 @@ class Engine:
         _check(self.lib.elp_launch_info(self.h, C.byref(a), C.byref(b), C.byref(c)), self.lib)
         return {"threads": a.value, "blocks": b.value, "lds_bytes": c.value}
 
+    def set_threads(self, threads):
+        _check(self.lib.elp_set_threads(self.h, int(threads)), self.lib)
+
+
+class Batch:
+    """Independent replicas of any boxes in one device buffer; one launch per time
+    unit covers every tile of every replica (elp_batch_* in include/elp_gpu.h)."""
+
+    def __init__(self, kind, boxes, tile=(16, 16, 16), sublattice=SUBLAT_DENSE, n_iter=13, device=0):
+        self.lib = load_elp()
+        self.kind = kind
+        self.boxes = [tuple(int(x) for x in (b if hasattr(b, "__len__") else (b, b, b))) for b in boxes]
+        self.n_rep = len(self.boxes)
+        self.tile = tuple(int(x) for x in tile)
+        self.dtype = np.uint8 if kind == "ring" else np.uint16
+        bx = np.ascontiguousarray(self.boxes, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        fn = self.lib.elp_ring_batch_create if kind == "ring" else self.lib.elp_lin_batch_create
+        _check(fn(C.byref(h), device, self.n_rep, bx.ctypes.data, *self.tile, int(sublattice), int(n_iter)), self.lib)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.elp_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def upload(self, rep, lat):
+        a = np.ascontiguousarray(lat, dtype=self.dtype)
+        _check(self.lib.elp_batch_upload(self.h, int(rep), a.ctypes.data, a.nbytes), self.lib)
+
+    def download(self, rep):
+        L = self.boxes[rep]
+        a = np.empty(L[0] * L[1] * L[2], dtype=self.dtype)
+        _check(self.lib.elp_batch_download(self.h, int(rep), a.ctypes.data, a.nbytes), self.lib)
+        return a
+
+    def run(self, origins, tau0, seeds, streams=None, units_per_graph=0, sync=True):
+        """origins: (n_time, n_rep, 3); tau0, seeds, streams: one per replica."""
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, self.n_rep, 3)
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(tau0, dtype=np.uint64), (self.n_rep,)))
+        streams = np.zeros(self.n_rep, dtype=np.uint32) if streams is None else streams
+        k = np.empty((self.n_rep, 2), dtype=np.uint32)
+        k[:, 0] = np.asarray(seeds, dtype=np.uint64) & 0xFFFFFFFF
+        k[:, 1] = np.asarray(streams, dtype=np.uint64) & 0xFFFFFFFF
+        _check(self.lib.elp_batch_run(self.h, len(o), o.ctypes.data, t.ctypes.data, k.ctypes.data,
+                                      int(units_per_graph)), self.lib)
+        if sync:
+            self.sync()
+
+    def sync(self):
+        _check(self.lib.elp_batch_sync(self.h), self.lib)
+
+    def last_run_ms(self):
+        ms = C.c_float(0)
+        _check(self.lib.elp_batch_last_run_ms(self.h, C.byref(ms)), self.lib)
+        return ms.value
+
+    def launch_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_uint64()
+        _check(self.lib.elp_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)), self.lib)
+        return {"threads": a.value, "blocks": b.value, "sites": c.value}
+
+    def set_threads(self, threads):
+        _check(self.lib.elp_batch_set_threads(self.h, int(threads)), self.lib)
+
 
 class _Host:
     """Handle API of a drop-in host library (fresh start, shift stream, tuv)."""
@@ -146,13 +224,18 @@ class _Host:
     libname = None
     dtype = None
 
-    def __init__(self, N=None, L=None, density=1.2, seed=0, path=None):
+    def __init__(self, N=None, L=None, density=1.2, seed=0, path=None, box_unit=0):
+        """box_unit: box edges are truncated to multiples of it (0 = 32, the
+        reference's rule; 16 allows boxes such as 48 for 16^3-tile engines)."""
         self.lib = _load(self.libname)
         p = self.prefix
         lib = self.lib
         if not getattr(lib, "_host_typed", False):
             getattr(lib, p + "new").restype = C.c_void_p
             getattr(lib, p + "new").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint32]
+            getattr(lib, p + "new_unit").restype = C.c_void_p
+            getattr(lib, p + "new_unit").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint32,
+                                                     C.c_int]
             getattr(lib, p + "from_file").restype = C.c_void_p
             getattr(lib, p + "from_file").argtypes = [C.c_char_p, C.c_uint32]
             getattr(lib, p + "free").argtypes = [C.c_void_p]
@@ -171,7 +254,8 @@ class _Host:
             h = getattr(lib, p + "from_file")(os.fsencode(path), int(seed))
         else:
             Ls = L if hasattr(L, "__len__") else (L, L, L)
-            h = getattr(lib, p + "new")(int(N), int(Ls[0]), int(Ls[1]), int(Ls[2]), float(density), int(seed))
+            h = getattr(lib, p + "new_unit")(int(N), int(Ls[0]), int(Ls[1]), int(Ls[2]), float(density), int(seed),
+                                             int(box_unit))
         if not h:
             raise ElpError(f"{p}new/from_file failed")
         self.h = C.c_void_p(h)

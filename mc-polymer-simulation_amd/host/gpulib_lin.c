@@ -104,16 +104,25 @@ int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULib
 /* ---------------- handle API (bench.py, tests, other drivers) ---------------- */
 typedef struct elp_lin_host { SimProperties sp; SimState ss; } elp_lin_host;
 
-elp_lin_host* elp_lin_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
+/* Fresh start with box edges truncated to multiples of box_unit (0: 32, as the
+   reference; a smaller unit, e.g. 16, allows boxes such as 48 for engines with
+   16^3 tiles, with a seed table sized by that unit). */
+elp_lin_host* elp_lin_host_new_unit(int N, int LT, int LU, int LV, double density, uint32_t seed, int box_unit) {
+    if (box_unit < 0 || (box_unit && box_unit % 4)) return NULL;
     elp_lin_host* h = (elp_lin_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.polLength = N; h->sp.density = density; h->sp.seed = seed; h->sp.nDevices = 1;
+    h->sp.boxUnit = box_unit;
     SetBoxDimension(&h->sp, LT, LU, LV);
     SimStateInit(&h->ss, &h->sp);
     GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
     RedistribSL(&h->ss, &h->sp);
     if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
     return h;
+}
+
+elp_lin_host* elp_lin_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
+    return elp_lin_host_new_unit(N, LT, LU, LV, density, seed, 0);
 }
 
 elp_lin_host* elp_lin_host_from_file(const char* path, uint32_t seed) {

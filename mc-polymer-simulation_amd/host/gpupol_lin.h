@@ -73,6 +73,7 @@ typedef struct SimProperties {
     uint* trans;
     uint maxPolLength;
     double density;
+    int boxUnit;       /* box edges are truncated to multiples of this; 0 = the reference's 32 */
 } SimProperties;
 
 typedef struct GPULibContext { int devIds[8]; } GPULibContext;

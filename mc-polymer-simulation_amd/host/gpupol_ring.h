@@ -70,6 +70,7 @@ typedef struct SimProperties {
     uint* trans;
     uint maxPolLength;
     double density;
+    int boxUnit;       /* box edges are truncated to multiples of this; 0 = the reference's 32 */
 } SimProperties;
 
 /* gpupol2/gpupol_cuda_def.h:8-20 (device buffers replaced by an engine) */

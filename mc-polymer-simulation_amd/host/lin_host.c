@@ -58,7 +58,7 @@ void SimStateFree(SimState* ss) {
 
 /* gpupol/gpupol_init.c:46-65 */
 void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
-    const int E = ELP_LIN_TILE;
+    const int E = sp->boxUnit > 0 ? sp->boxUnit : ELP_LIN_TILE;   /* 0: the reference's 32 */
     sp->nwt = LT / E; sp->nwu = LU / E; sp->nwv = LV / E;
     sp->nwg = (size_t)sp->nwt * sp->nwu * sp->nwv;
     sp->LT = E * sp->nwt; sp->LU = E * sp->nwu; sp->LV = E * sp->nwv;

@@ -53,9 +53,10 @@ void SimStateFree(SimState* ss) {
 /* gpupol2/gpupol_init.c:39-41 */
 void ClearLattice(SimState* ss, SimProperties* sp) { memset(ss->lattice, 0, sp->latSize); }
 
-/* gpupol2/gpupol_init.c:43-62 -- box edges truncated to multiples of 32 */
+/* gpupol2/gpupol_init.c:43-62 -- box edges truncated to multiples of 32
+   (of sp->boxUnit when set: boxes such as 48 for tiles of 16) */
 void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
-    const int E = ELP_RING_TILE;
+    const int E = sp->boxUnit > 0 ? sp->boxUnit : ELP_RING_TILE;
     if (LT % E || LU % E || LV % E) printf("Warning changing box dimensions to be a multiple of 24!\n");
     sp->nwt = LT / E; sp->nwu = LU / E; sp->nwv = LV / E;
     sp->nwg = (size_t)sp->nwt * sp->nwu * sp->nwv;

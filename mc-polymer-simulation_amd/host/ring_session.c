@@ -16,21 +16,30 @@ typedef struct elp_ring_host {
     long cumT, cumU, cumV;
 } elp_ring_host;
 
-/* Fresh start (no files): box L^3 truncated to a multiple of 32, rings of N
+/* Fresh start (no files): box edges truncated to multiples of box_unit (0: 32,
+   as the reference; a smaller unit, e.g. 16, allows boxes such as 48 for
+   engines with 16^3 tiles, with a seed table sized by that unit), rings of N
    monomers at the given density and seed.  Returns NULL on failure. */
-elp_ring_host* elp_ring_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
+elp_ring_host* elp_ring_host_new_unit(int N, int LT, int LU, int LV, double density, uint32_t seed, int box_unit) {
+    if (box_unit < 0 || (box_unit && box_unit % 4)) return NULL;
     elp_ring_host* h = (elp_ring_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.polLength = N;
     h->sp.density = density;
     h->sp.seed = seed;
     h->sp.nDevices = 1;
+    h->sp.boxUnit = box_unit;
     SetBoxDimension(&h->sp, LT, LU, LV);
     SimStateInit(&h->ss, &h->sp);
     GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
     RedistribSL(&h->ss, &h->sp);
     if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
     return h;
+}
+
+/* Fresh start with the reference's box rule (edges truncated to multiples of 32). */
+elp_ring_host* elp_ring_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
+    return elp_ring_host_new_unit(N, LT, LU, LV, density, seed, 0);
 }
 
 /* Start from a tuv file (resume path of LoadPolymers, without the seed bump). */
