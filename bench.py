@@ -78,6 +78,14 @@ def bench_batch(args, tile):
     Ls, Ns = [32, 48, 64, 96], [50, 200, 1000, 2000]
     hosts = [elp.RingHost(N=Ns[(r // 4) % 4], L=Ls[r % 4], density=args.density, seed=r, box_unit=16)
              for r in range(64)]
+    G = args.units_per_graph
+    tuned = None
+    if args.autotune:
+        sched, log = elp.autotune("ring", [h.lattice() for h in hosts], [h.L for h in hosts], time_units=20)
+        print(log, file=sys.stderr, end="")
+        tuned = sched.record()
+        tile, dense, n_iter, G = sched.tile, sched.sublattice == elp.SUBLAT_DENSE, sched.n_iter, sched.units_per_graph
+        args.sublattice, args.threads = ("dense" if dense else "cubic"), sched.threads
     bat = elp.Batch("ring", [h.L for h in hosts], tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC, n_iter)
     if args.threads:
         bat.set_threads(args.threads)
@@ -89,7 +97,6 @@ def bench_batch(args, tile):
     info = bat.launch_info()
     nsite = info["sites"]
     attempts_per_step = n_iter * 4 * (nsite // 64) * (4 if dense else 1)
-    G = args.units_per_graph
     tau = 0
     if args.warmup:
         org = np.stack([h.origins(tile, args.warmup) for h in hosts], axis=1)
@@ -128,6 +135,7 @@ def bench_batch(args, tile):
             "sublattice": args.sublattice,
             "iterations_per_time_unit": n_iter,
             "units_per_graph": G,
+            "autotuned_schedule": tuned,
             "threads_per_block": info["threads"],
             "blocks_per_launch": info["blocks"],
             "parallelism": "single GPU",
@@ -171,6 +179,8 @@ def main():
     ap.add_argument("--units-per-graph", type=int, default=16,
                     help="batch: time units captured per HIP graph (0 = direct launches)")
     ap.add_argument("--threads", type=int, default=0, help="threads per workgroup (0 = schedule default)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="batch: 1 = take tile, sub-lattice, threads and units per graph from elp_autotune")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -142,6 +142,34 @@ int elp_batch_last_run_ms(elp_batch* b, float* ms);
 int elp_batch_info(elp_batch* b, int* threads_per_block, int* blocks_per_time_unit, uint64_t* sites);
 int elp_batch_set_threads(elp_batch* b, int threads);
 
+/*
+ * Schedule autotuner (no reference counterpart: the reference fixes its
+ * tiles and iterations at compile time, gpupol2/gpupol_init.c:76).
+ * A schedule = tile shape, sub-lattice, iterations per time unit, threads
+ * per workgroup and time units per graph launch; every choice of it gives a
+ * trajectory that the C replay reproduces for that schedule.
+ */
+typedef struct elp_schedule {
+    int tt, tu, tv;
+    int sublattice;
+    int n_iter;
+    int threads;
+    int units_per_graph;
+    double attempts_per_s;   /* measured by the tuner */
+} elp_schedule;
+
+/* Time every compiled batched configuration whose tile divides all boxes
+   on scratch copies of lattices[r] (kind 0 ring, 1 linear; time_units timed
+   per candidate), then the time units per graph for the winner.  The
+   lattices are not modified.  log (optional) receives one line per
+   candidate. */
+int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const void* const* lattices,
+                 int time_units, elp_schedule* best, char* log, uint64_t log_len);
+/* One-line text record of a schedule, and its parser (round trip exact
+   except attempts_per_s, printed with 7 significant digits). */
+int elp_schedule_format(const elp_schedule* s, char* buf, uint64_t len);
+int elp_schedule_parse(const char* text, elp_schedule* s);
+
 /* Diagnostic (no reference counterpart): device-to-device copy of nbytes
    (multiple of 16) with a plain streaming kernel, reps times; *ms = average
    time per copy.  Used to calibrate HBM byte counters and to measure the

@@ -334,6 +334,11 @@ static int batch_create(elp_batch** out, int kind, int device, int n_rep, const 
     b->n_sites = site;
     b->n_tiles = (uint32_t)tile;
     b->uploaded.assign(n_rep, 0);
+    // the default launch shape, else the widest compiled one
+    for (int th : {1024, 512, 256, 128, 64}) {
+        if (batch_launch_one(b, true, 0) == 0) break;
+        b->threads = th;
+    }
     if (batch_launch_one(b, true, 0) != 0) {
         int th = b->threads;
         delete b;

@@ -12,6 +12,8 @@ from .core import (  # noqa: F401
     SUBLAT_CUBIC,
     SUBLAT_DENSE,
     ElpError,
+    Schedule,
+    autotune,
     Batch,
     Engine,
     RingHost,

@@ -22,6 +22,41 @@ class ElpError(RuntimeError):
 _libs = {}
 
 
+class Schedule(C.Structure):
+    """elp_schedule of include/elp_gpu.h."""
+    _fields_ = [("tt", C.c_int), ("tu", C.c_int), ("tv", C.c_int), ("sublattice", C.c_int), ("n_iter", C.c_int),
+                ("threads", C.c_int), ("units_per_graph", C.c_int), ("attempts_per_s", C.c_double)]
+
+    @property
+    def tile(self):
+        return (self.tt, self.tu, self.tv)
+
+    def record(self):
+        buf = C.create_string_buffer(256)
+        _check(load_elp().elp_schedule_format(C.byref(self), buf, len(buf)))
+        return buf.value.decode()
+
+    @classmethod
+    def parse(cls, text):
+        s = cls()
+        _check(load_elp().elp_schedule_parse(os.fsencode(text), C.byref(s)))
+        return s
+
+
+def autotune(kind, lattices, boxes, time_units=20, device=0):
+    """Fastest compiled schedule for these replicas (elp_autotune); returns
+    (Schedule, log text).  The lattices are only read."""
+    lib = load_elp()
+    arrs = [np.ascontiguousarray(a, dtype=np.uint8 if kind == "ring" else np.uint16) for a in lattices]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    bx = np.ascontiguousarray(boxes, dtype=np.int32).reshape(-1)
+    best = Schedule()
+    log = C.create_string_buffer(8192)
+    _check(lib.elp_autotune(device, 0 if kind == "ring" else 1, len(arrs), bx.ctypes.data, ptrs, int(time_units),
+                            C.byref(best), log, len(log)), lib)
+    return best, log.value.decode()
+
+
 def _load(name):
     if name in _libs:
         return _libs[name]
@@ -60,6 +95,9 @@ def load_elp():
         lib.elp_batch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(U64)]
         lib.elp_batch_set_threads.argtypes = [P, I]
         lib.elp_set_threads.argtypes = [P, I]
+        lib.elp_autotune.argtypes = [I, I, I, P, P, I, C.POINTER(Schedule), C.c_char_p, U64]
+        lib.elp_schedule_format.argtypes = [C.POINTER(Schedule), C.c_char_p, U64]
+        lib.elp_schedule_parse.argtypes = [C.c_char_p, C.POINTER(Schedule)]
         lib._elp_typed = True
     return lib
 

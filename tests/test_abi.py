@@ -44,6 +44,34 @@ def test_error_paths_without_gpu_calls():
     assert lib.elp_run(None, 1, None, 0, 0, 0) == -1
 
 
+def test_batch_error_paths_without_gpu_calls():
+    lib = elp.load_elp()
+    h = C.c_void_p()
+    boxes = (C.c_int32 * 6)(32, 32, 32, 40, 32, 32)
+    assert lib.elp_ring_batch_create(C.byref(h), 0, 2, boxes, 16, 16, 16, 1, 13) == -1
+    assert b"replica 1" in lib.elp_last_error()
+    boxes = (C.c_int32 * 3)(32, 32, 32)
+    assert lib.elp_lin_batch_create(C.byref(h), 0, 0, boxes, 16, 16, 16, 1, 16) == -1
+    assert lib.elp_batch_run(None, 1, None, None, None, 0) == -1
+
+
+def test_schedule_record_round_trip():
+    s = elp.Schedule(16, 16, 16, elp.SUBLAT_DENSE, 13, 256, 16, 5.25e11)
+    rec = s.record()
+    assert rec == "tile=16x16x16 sublattice=dense n_iter=13 threads=256 units_per_graph=16 attempts_per_s=5.250000e+11"
+    t = elp.Schedule.parse(rec)
+    assert (t.tile, t.sublattice, t.n_iter, t.threads, t.units_per_graph) == ((16, 16, 16), 1, 13, 256, 16)
+    assert t.attempts_per_s == 5.25e11
+    u = elp.Schedule.parse("tile=32x32x32 sublattice=cubic n_iter=51 threads=256 units_per_graph=0")
+    assert (u.tile, u.sublattice, u.n_iter) == ((32, 32, 32), 0, 51)
+    for bad in ("tile=16x16 sublattice=dense", "tile=16x16x16 sublattice=sparse n_iter=1 threads=64 units_per_graph=0"):
+        try:
+            elp.Schedule.parse(bad)
+        except elp.ElpError:
+            continue
+        raise AssertionError(bad)
+
+
 def test_drop_in_executables_print_usage():
     for exe in ("gpupol3_cuda_ring", "gpupol_cuda_lin"):
         r = subprocess.run([os.path.join(elp.PKG_DIR, "bin", exe)], capture_output=True, text=True)

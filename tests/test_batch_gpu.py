@@ -109,6 +109,34 @@ def test_ring_batch_64_replicas_full_size():
     bat.close()
 
 
+@pytest.mark.parametrize("kind", ["ring", "linear"])
+def test_autotuned_schedule_replays_bit_exact(kind):
+    """The tuner's record, parsed back, drives a batch whose replicas equal the
+    C replay of that recorded schedule."""
+    _say(f"[autotune] {kind}")
+    host_cls, replay = (elp.RingHost, replay_ring) if kind == "ring" else (elp.LinHost, replay_lin)
+    spec = [(100, 32, 11), (200, 64, 12)]
+    hosts = [host_cls(N=N, L=L, density=1.2, seed=s) for N, L, s in spec]
+    lats = [h.lattice() for h in hosts]
+    best, log = elp.autotune(kind, lats, [h.L for h in hosts], time_units=8)
+    _say(log)
+    assert all(np.array_equal(h.lattice(), l) for h, l in zip(hosts, lats))     # inputs untouched
+    assert best.attempts_per_s > 0 and "chosen:" in log
+    s = elp.Schedule.parse(best.record())
+    bat = elp.Batch(kind, [h.L for h in hosts], s.tile, s.sublattice, s.n_iter)
+    bat.set_threads(s.threads)
+    for r, h in enumerate(hosts):
+        bat.upload(r, lats[r])
+        h.load_shift(s.tile)
+    n_time = 2 * max(s.units_per_graph, 1) + 1
+    org = np.stack([h.origins(s.tile, n_time) for h in hosts], axis=1)
+    bat.run(org, tau0=5, seeds=[h.seed for h in hosts], streams=[0, 0], units_per_graph=s.units_per_graph)
+    for r, h in enumerate(hosts):
+        want = replay(lats[r], h.L, s.tile, s.sublattice, s.n_iter, org[:, r], 5, h.seed, 0)
+        assert np.array_equal(bat.download(r), want), f"replica {r}"
+    bat.close()
+
+
 def test_batch_rejects_bad_input():
     with pytest.raises(elp.ElpError, match="not divisible"):
         elp.Batch("ring", [(32, 32, 32), (40, 32, 32)], (16, 16, 16), elp.SUBLAT_DENSE, 13)
