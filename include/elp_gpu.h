@@ -106,6 +106,25 @@ int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_
 int elp_set_threads(elp_engine* e, int threads);
 
 /*
+ * Move rates of the ring engine.
+ *   ELP_RULES_REFERENCE (default): ringpol3's rates (gpupol2/ringpol3.cu:92-218).
+ *     A TransForw that splits stored length off a bond whose two ends both
+ *     hold it has rate 1/4 per iteration, the forced TransBack that undoes it
+ *     rate 1; the move set is not reversible, and its equilibrium depends on
+ *     the schedule (cubic vs dense sub-lattice).
+ *   ELP_RULES_BALANCED: the forced TransBack is accepted with probability 1/4
+ *     (random bits 1-2 of its byte zero), and every slot of every substep
+ *     draws its move from bits 6-7 of its byte (TransForw 1/2, DiffuseSL
+ *     1/4, TransBack 1/4) instead of the fixed sequence.  Every move then
+ *     has the probability of its reverse within one substep, each substep
+ *     preserves the uniform measure over lattice states, and every schedule
+ *     samples it.
+ * The linear engine has the reference rates only.
+ */
+enum { ELP_RULES_REFERENCE = 0, ELP_RULES_BALANCED = 1 };
+int elp_set_rules(elp_engine* e, int rules);
+
+/*
  * Batched replicas (no reference counterpart: the reference runs one
  * independent system per process or per device, gpupol2/gpupol_init.c:14-17,
  * gpupol2/gpupol2.c:52-55).  n_rep independent lattices of any boxes
@@ -141,6 +160,7 @@ int elp_batch_sync(elp_batch* b);
 int elp_batch_last_run_ms(elp_batch* b, float* ms);
 int elp_batch_info(elp_batch* b, int* threads_per_block, int* blocks_per_time_unit, uint64_t* sites);
 int elp_batch_set_threads(elp_batch* b, int threads);
+int elp_batch_set_rules(elp_batch* b, int rules);
 
 /*
  * Schedule autotuner (no reference counterpart: the reference fixes its

@@ -72,6 +72,7 @@ struct elp_engine {
     bool uploaded;
     bool timed;
     uint32_t trans[4];
+    uint32_t rules;        // ELP_RULES_*
 };
 
 static int launch_one(elp_engine* e, bool probe, const elp::RingLaunch& p) {
@@ -204,6 +205,7 @@ int elp_run(elp_engine* e, int n_time, const int32_t* origins, uint64_t tau0, ui
     p.ntile_t = e->LT / e->tt; p.ntile_u = e->LU / e->tu;
     p.seed = seed; p.stream = stream;
     p.n_iter = e->n_iter;
+    p.rules = e->rules;
     for (int i = 0; i < 4; i++) p.trans[i] = e->trans[i];
     HIPCHK(hipEventRecord(e->ev0, e->stream));
     for (int i = 0; i < n_time; i++) {
@@ -275,10 +277,11 @@ struct elp_batch {
     hipGraphExec_t gexec = nullptr;        // cached graph of g_units launches + cursor advance
     int g_units = 0;
     const void* g_steps = nullptr;
+    uint32_t rules = 0;                    // ELP_RULES_*
 };
 
 static int batch_launch_one(elp_batch* b, bool probe, int k) {
-    elp::BatchLaunch bl{b->d_reps, b->d_rep_of_tile, b->d_steps, b->d_cursor, k, b->n_rep, b->n_iter};
+    elp::BatchLaunch bl{b->d_reps, b->d_rep_of_tile, b->d_steps, b->d_cursor, k, b->n_rep, b->n_iter, b->rules};
     const bool dense = b->sublattice == ELP_SUBLAT_DENSE;
     if (b->kind == 0)
         return elp::ring_batch_launch(b->tt, b->tu, b->tv, dense, b->threads, probe, (int)b->n_tiles, b->stream,
@@ -547,6 +550,27 @@ int elp_batch_set_threads(elp_batch* b, int threads) {
         (void)hipGraphExecDestroy(b->gexec);
         b->gexec = nullptr;
     }
+    return ELP_OK;
+}
+
+int elp_set_rules(elp_engine* e, int rules) {
+    if (!e) return fail(ELP_E_ARG, "NULL engine");
+    if (rules != ELP_RULES_REFERENCE && !(rules == ELP_RULES_BALANCED && e->kind == 0))
+        return fail(ELP_E_ARG, "rules %d not available for the %s engine", rules, e->kind == 0 ? "ring" : "linear");
+    e->rules = (uint32_t)rules;
+    return ELP_OK;
+}
+
+int elp_batch_set_rules(elp_batch* b, int rules) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    if (rules != ELP_RULES_REFERENCE && !(rules == ELP_RULES_BALANCED && b->kind == 0))
+        return fail(ELP_E_ARG, "rules %d not available for the %s batch", rules, b->kind == 0 ? "ring" : "linear");
+    if ((uint32_t)rules != b->rules && b->gexec) {      // captured launches carry the old rules
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    b->rules = (uint32_t)rules;
     return ELP_OK;
 }
 

@@ -15,6 +15,7 @@ struct RingLaunch {
     uint32_t seed, stream;   // Philox key
     int n_iter;              // iterations of the 4-substep sequence
     uint32_t trans[4];       // first bond of each decomposition (SetGPUTrans)
+    uint32_t rules;          // ring move rates: 0 reference, 1 balanced (ELP_RULES_*)
 };
 
 using LinLaunch = RingLaunch;
@@ -46,6 +47,7 @@ struct BatchLaunch {
     int k;
     int n_rep;
     int n_iter;
+    uint32_t rules;
 };
 
 // Launch one time unit (nblocks = number of tiles).  probe=true only checks

@@ -274,8 +274,12 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
     if (!moveFirst) lat[na] = (uint8_t)ns;
 }
 
+// rules = 1 (balanced): a fold forced by stored length at exactly one end
+// (reference rate 1) is accepted with probability 1/4 (r bits 1-2 zero), the
+// rate of the TransForw that undoes it, so every move and its reverse have
+// equal probabilities (with the per-slot move choice of RingMoves::move).
 template <class G>
-__device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r) {
+__device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -290,6 +294,7 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     uint32_t ws = lat[wa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
     if (!s || G::out(Ps | Pw) || (ss & 0x40u) || !combine || newNext == 0x3u || newNext == 0xcu || (sl && wsl)) return;
+    if (rules && (sl | wsl) && (r & 6u)) return;
     const bool moveFirst = (!sl && !wsl) ? (r & 1u) : (sl == 0);
     const uint32_t srcLabel = ss & 0x30u;
     if (moveFirst) {
@@ -330,9 +335,20 @@ struct RingMoves {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
     template <class G>
-    __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r) {
+    __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
+        if (rules) {
+            // balanced rates: every slot draws its move from bits 6-7 of its
+            // byte (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the
+            // proportions of the reference sequence), so each substep is a
+            // symmetric kernel and preserves the uniform measure by itself
+            const uint32_t ty = r >> 6;
+            if (ty < 2u) trans_forw2<G>(lat, P, r);
+            else if (ty == 2u) diffuse_sl2<G>(lat, P);
+            else trans_back2<G>(lat, P, r, rules);
+            return;
+        }
         if (s == 1) diffuse_sl2<G>(lat, P);
-        else if (s == 3) trans_back2<G>(lat, P, r);
+        else if (s == 3) trans_back2<G>(lat, P, r, rules);
         else trans_forw2<G>(lat, P, r);
     }
 };
@@ -368,6 +384,7 @@ static bool launch_padded(int variant, int nblocks, hipStream_t stream, uint8_t*
 #define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                              \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
+        if (variant == 1 && p.rules) return (int)hipErrorInvalidValue; /* first version: reference rates */ \
         if (variant == 1)                                                                                  \
             hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, stream, \
                                glat, p);                                                                   \

@@ -6,7 +6,7 @@
 // set's substeps with a workgroup barrier after each substep, and writes the
 // tile back in place.  The move set M supplies the site type, the number of
 // substeps per iteration and the move of substep s:
-//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r).
+//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules).
 //
 // Randomness (Philox4x32-10, key = (seed, stream)):
 //   site offsets : counter (tau, tile, 0xFFFFFFFF, iter/4), word iter%4,
@@ -27,10 +27,12 @@ struct UnitArgs {
     int ot, ou, ov;          // tile-grid origin of this time unit
     uint32_t tau, seed, stream;
     int n_iter;
+    uint32_t rules;          // move rates (ring: 0 reference, 1 balanced)
 };
 
 __device__ __forceinline__ UnitArgs unit_args(const RingLaunch& p) {
-    return UnitArgs{p.LT, p.LU, p.LV, p.ntile_t, p.ntile_u, p.ot, p.ou, p.ov, p.tau, p.seed, p.stream, p.n_iter};
+    return UnitArgs{p.LT, p.LU, p.LV, p.ntile_t, p.ntile_u, p.ot, p.ou, p.ov, p.tau, p.seed, p.stream, p.n_iter,
+                    p.rules};
 }
 
 template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
@@ -90,7 +92,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                     uint32_t P;
                     if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
                     else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                    M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu);
+                    M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu, p.rules);
                 }
                 __syncthreads();
             }
@@ -120,7 +122,7 @@ __device__ __forceinline__ void batch_unit(typename M::site_t* __restrict__ gbas
     const BatchRep rep = b.reps[r];
     const BatchStep st = b.steps[(size_t)(*b.cursor + b.k) * (size_t)b.n_rep + r];
     const UnitArgs p{rep.LT, rep.LU, rep.LV, rep.ntile_t, rep.ntile_u, st.ot, st.ou, st.ov,
-                     st.tau, rep.seed, rep.stream, b.n_iter};
+                     st.tau, rep.seed, rep.stream, b.n_iter, b.rules};
     tile_unit<M, TT, TU, TV, DENSE, THREADS, PAD>(gbase + rep.site_off, tile, p, g - rep.tile0);
 }
 

@@ -11,6 +11,8 @@ from .core import (  # noqa: F401
     LIB_DIR,
     SUBLAT_CUBIC,
     SUBLAT_DENSE,
+    RULES_REFERENCE,
+    RULES_BALANCED,
     ElpError,
     Schedule,
     autotune,

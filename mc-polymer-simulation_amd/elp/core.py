@@ -13,6 +13,8 @@ HEADER = os.path.join(REPO_DIR, "include", "elp_gpu.h")
 
 SUBLAT_CUBIC = 0
 SUBLAT_DENSE = 1
+RULES_REFERENCE = 0      # ringpol3's move rates
+RULES_BALANCED = 1       # forced TransBack at rate 1/4: uniform measure stationary
 
 
 class ElpError(RuntimeError):
@@ -95,6 +97,8 @@ def load_elp():
         lib.elp_batch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(U64)]
         lib.elp_batch_set_threads.argtypes = [P, I]
         lib.elp_set_threads.argtypes = [P, I]
+        lib.elp_batch_set_rules.argtypes = [P, I]
+        lib.elp_set_rules.argtypes = [P, I]
         lib.elp_autotune.argtypes = [I, I, I, P, P, I, C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_schedule_format.argtypes = [C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_schedule_parse.argtypes = [C.c_char_p, C.POINTER(Schedule)]
@@ -190,6 +194,10 @@ class Engine:
     def set_threads(self, threads):
         _check(self.lib.elp_set_threads(self.h, int(threads)), self.lib)
 
+    def set_rules(self, rules):
+        """ring move rates: RULES_REFERENCE (default) or RULES_BALANCED"""
+        _check(self.lib.elp_set_rules(self.h, int(rules)), self.lib)
+
 
 class Batch:
     """Independent replicas of any boxes in one device buffer; one launch per time
@@ -253,6 +261,9 @@ class Batch:
 
     def set_threads(self, threads):
         _check(self.lib.elp_batch_set_threads(self.h, int(threads)), self.lib)
+
+    def set_rules(self, rules):
+        _check(self.lib.elp_batch_set_rules(self.h, int(rules)), self.lib)
 
 
 class _Host:

@@ -1,0 +1,72 @@
+/*
+ * elp_stats.h -- polymer statistics of ELP configurations (libelp_stats.so).
+ *
+ * A restatement of the reference analysis (analysis/mode_io.c:510-600 reading
+ * and unwrapping, analysis/mode_compute.c:145-217 SL ratio / CMS / Rg^2,
+ * :429-504 contact probability, :506-570 monomer and CMS mean-square
+ * displacements, analysis/mode_io.c:54-70 normalisation), for one system:
+ * n_chain chains of n_mono monomers observed over a series of frames.
+ *
+ * Frames come from tuv files (the format of gpupol2/gpupol_io.c:105-203 and
+ * cpupol/efpol_io.c:3-77) or from arrays (start site + bond codes per chain).
+ * Positions are unwrapped along each chain from its start site, the start is
+ * unwrapped against the previous frame by the minimum image, and the chain
+ * is then moved by whole boxes so that its mean position moves by less than
+ * half a box between frames (analysis/mode_io.c:531-581).
+ * Cartesian coordinates x = t+u-v, y = t-u, z = v; |bond|^2 = 2, so lengths
+ * are reported in bond units (divided by sqrt 2) as the reference does.
+ */
+#ifndef ELP_STATS_H
+#define ELP_STATS_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct elp_series elp_series;
+
+/* ring: 1 for rings (n_mono bonds close the chain), 0 for linear chains
+   (n_mono-1 bonds, the tuv text ends with 'f'). */
+elp_series* elp_series_new(int ring, int LT, int LU, int LV, int n_chain, int n_mono, int max_frames);
+void elp_series_free(elp_series* s);
+
+/* Add one frame: chain c starts at starts[3c..3c+2] (t,u,v) and has bond codes
+   bonds[c*stride + i], i < n_mono (0 = stored length; linear: the last entry
+   is ignored).  Returns 0, or -1 when the series is full / sizes disagree. */
+int elp_series_add(elp_series* s, const int32_t* starts, const uint8_t* bonds, int stride);
+
+/* Add one frame read from a tuv file; every chain must have n_mono monomers.
+   Returns 0 or a negative code (-1 full, -2 unreadable, -3 shape mismatch). */
+int elp_series_add_tuv(elp_series* s, const char* path);
+
+int elp_series_frames(const elp_series* s);
+
+/* Unwrapped (t,u,v) of chain c in frame f: out[3*i + 0..2], i < n_mono. */
+int elp_series_positions(const elp_series* s, int f, int c, int32_t* out);
+
+/* Squared radius of gyration of every chain in frame f (bond units). */
+int elp_series_rg2(const elp_series* s, int f, double* rg2);
+
+/*
+ * Equilibrium statics over frames n_therm.. and all chains:
+ *   rg2    mean squared radius of gyration (ComputeGyration)
+ *   slrat  mean fraction of bonds with both ends on one site (ComputeSL)
+ *   pc     pc[g], g < n_mono: contact probability of monomers g apart along
+ *          the chain (same or nearest-neighbour site, same periodic image),
+ *          normalised by n_chain * frames * (n_mono - g) (WriteAvgContactProbability)
+ */
+int elp_series_statics(const elp_series* s, int n_therm, double* rg2, double* slrat, double* pc);
+
+/*
+ * Mean-square displacements over frames n_therm.. at lag k frames, k < n_out:
+ *   g1[k]  middle monomer (index polSize/2), minus the drift of the whole
+ *          system's mean position, |dr|^2 / 2 (AddMono)
+ *   g3[k]  chain centre of mass, minus the same drift (AddCMS)
+ * averaged over all time origins and chains.
+ */
+int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -95,7 +95,7 @@ static uint32_t xorshift128(uint32_t* s) {      /* s = (x, y, z, w) of a uint4 *
 static uint32_t draw(Draw* d) { return d->ref ? xorshift128(d->xs) : d->fixed; }
 
 /* ---------------- moves on a tile buffer ---------------- */
-typedef struct { uint8_t* b; int T, U, V; uint32_t tr[4]; } Tile;
+typedef struct { uint8_t* b; int T, U, V; uint32_t tr[4]; int rules; } Tile;   /* rules: 0 reference, 1 balanced */
 
 /* index of (t,u,v)+bond, or -1 if it leaves the tile */
 static int step(const Tile* X, int t, int u, int v, uint32_t bond, int* nt, int* nu, int* nv) {
@@ -153,6 +153,9 @@ static void trans_back(Tile* X, int t, int u, int v, Draw* rd) {
     uint32_t ws = X->b[w], wsl = ws & 0x40;
     if (sl + wsl == 0x80) return;
     uint32_t r = draw(rd);
+    /* balanced rates: a fold forced by stored length at one end is accepted
+       with probability 1/4, the rate of the TransForw that undoes it */
+    if (X->rules && (sl | wsl) && (r & 6)) return;
     int moveFirst = (sl + wsl == 0) ? (int)(r & 1) : (sl == 0x40 ? 0 : 1);
     if (moveFirst) {
         s = nn | (lab << 1) | srcLab | 0x40;
@@ -185,6 +188,24 @@ static void diffuse_sl(Tile* X, int t, int u, int v) {
     X->b[n] = (uint8_t)ns;
 }
 
+/*
+ * One move on a T x U x V tile buffer (t fastest) with the random byte r:
+ * type 0 TransForw, 1 DiffuseSL, 2 TransBack; rules 0 reference, 1 balanced.  For enumerating transition
+ * probabilities of the move set (detailed-balance tests).  Returns 0, or -1
+ * on a bad type / site.
+ */
+int orc_ring_move(uint8_t* tile, int T, int U, int V, int type, int t, int u, int v, uint32_t r, int rules) {
+    if (t < 0 || t >= T || u < 0 || u >= U || v < 0 || v >= V) return -1;
+    Tile X = {tile, T, U, V, {0, 0, 0, 0}, rules};
+    gpu_trans(X.tr);
+    Draw d = {0, r, NULL};
+    if (type == 0) trans_forw(&X, t, u, v, &d);
+    else if (type == 1) diffuse_sl(&X, t, u, v);
+    else if (type == 2) trans_back(&X, t, u, v, &d);
+    else return -1;
+    return 0;
+}
+
 /* site of slot `slot` in coset c of the given sub-lattice */
 static void slot_site(int sublat, int T, int U, int slot, uint32_t c, int* t, int* u, int* v) {
     int ncT = T / 4, ncU = U / 4;
@@ -205,11 +226,13 @@ static void slot_site(int sublat, int T, int U, int slot, uint32_t c, int* t, in
  * order: 0 ascending slots, 1 descending, 2 shuffled (fixed LCG per substep).
  * Returns 0, or -1 on bad arguments.
  */
-int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, int sublat, int n_iter,
-                 int n_time, const int32_t* origins, uint64_t tau0, uint32_t seed, uint32_t stream, int order) {
+int orc_ring_run_rules(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, int sublat, int n_iter,
+                       int n_time, const int32_t* origins, uint64_t tau0, uint32_t seed, uint32_t stream, int order,
+                       int rules) {
     if (TT % 4 || TU % 4 || TV % 4 || LT % TT || LU % TU || LV % TV) return -1;
     Tile X;
     X.T = TT; X.U = TU; X.V = TV;
+    X.rules = rules;
     X.b = (uint8_t*)malloc((size_t)TT * TU * TV);
     gpu_trans(X.tr);
     int nT = LT / TT, nU = LU / TU, nV = LV / TV;
@@ -248,8 +271,12 @@ int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
                                 int slot = perm[kk], t, u, v;
                                 slot_site(sublat, TT, TU, slot, c, &t, &u, &v);
                                 Draw rd = {0, (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> (8 * s)) & 0xff, NULL};
-                                if (s == 1) diffuse_sl(&X, t, u, v);
-                                else if (s == 3) trans_back(&X, t, u, v, &rd);
+                                /* move of this slot: the substep's (TF, DSL, TF, TB), or under
+                                   balanced rates bits 6-7 of the byte: 0,1 TF, 2 DSL, 3 TB */
+                                int mv = s == 1 ? 1 : (s == 3 ? 2 : 0);
+                                if (rules) mv = (rd.fixed >> 6) < 2 ? 0 : (int)(rd.fixed >> 6) - 1;
+                                if (mv == 1) diffuse_sl(&X, t, u, v);
+                                else if (mv == 2) trans_back(&X, t, u, v, &rd);
                                 else trans_forw(&X, t, u, v, &rd);
                             }
                         }
@@ -280,11 +307,17 @@ int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
  * shift is needed mod L, not mod 32.  Judge: the reference kernel compiled unmodified
  * (oracle/refkernel_ring.hip).
  */
+int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, int sublat, int n_iter,
+                 int n_time, const int32_t* origins, uint64_t tau0, uint32_t seed, uint32_t stream, int order) {
+    return orc_ring_run_rules(lat, LT, LU, LV, TT, TU, TV, sublat, n_iter, n_time, origins, tau0, seed, stream, order, 0);
+}
+
 int orc_ring_run_ref(uint8_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {
     if (L % 32) return -1;
     const int nw = L / 32, W = 32;
     Tile X;
     X.T = X.U = X.V = W;
+    X.rules = 0;
     X.b = (uint8_t*)malloc((size_t)W * W * W);
     gpu_trans(X.tr);
     for (int i = 0; i < n_time; i++) {

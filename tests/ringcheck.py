@@ -85,11 +85,14 @@ def oracle():
     return _orc
 
 
-def replay_ring(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0):
+def replay_ring(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0, rules=0):
     lat = np.ascontiguousarray(lat, dtype=np.uint8).copy()
     o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
-    rc = oracle().orc_ring_run(lat.ctypes.data, *L, *tile, int(sublat), int(n_iter), len(o), o.ctypes.data,
-                               int(tau0), int(seed), int(stream), int(order))
+    lib = oracle()
+    lib.orc_ring_run_rules.argtypes = [C.c_void_p] + [C.c_int] * 9 + [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                                                       C.c_int, C.c_int]
+    rc = lib.orc_ring_run_rules(lat.ctypes.data, *L, *tile, int(sublat), int(n_iter), len(o), o.ctypes.data,
+                                int(tau0), int(seed), int(stream), int(order), int(rules))
     assert rc == 0
     return lat
 

@@ -43,6 +43,26 @@ def test_ring_bit_exact_vs_replay(tile, sublat, n_iter, seed):
     eng.close()
 
 
+@pytest.mark.parametrize("tile,sublat,n_iter", [SCHEDULES[0], SCHEDULES[2]])
+def test_ring_balanced_rules_bit_exact_vs_replay(tile, sublat, n_iter):
+    _say(f"[ring parity, balanced rates] tile={tile} sublat={sublat}")
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=9)
+    lat0 = h.lattice()
+    h.load_shift(tile)
+    org = h.origins(tile, 8)
+    eng = elp.Engine("ring", h.L, tile, sublat, n_iter)
+    eng.set_rules(elp.RULES_BALANCED)
+    eng.upload(lat0)
+    eng.run(org, tau0=3, seed=h.seed)
+    got = eng.download()
+    want = replay_ring(lat0, h.L, tile, sublat, n_iter, org, 3, h.seed, rules=elp.RULES_BALANCED)
+    ref = replay_ring(lat0, h.L, tile, sublat, n_iter, org, 3, h.seed)
+    assert np.count_nonzero(want != ref) > 0          # the rates change the trajectory
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
+    with pytest.raises(elp.ElpError):
+        elp.Engine("linear", (32, 32, 32), tile, sublat, n_iter).set_rules(elp.RULES_BALANCED)
+
+
 def test_ring_bit_exact_L64_noncubic_box():
     _say("[ring parity] L=(64,32,96) N=200 dense 16^3")
     h = elp.RingHost(N=200, L=(64, 32, 96), density=1.2, seed=7)
