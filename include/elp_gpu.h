@@ -190,6 +190,51 @@ int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const vo
 int elp_schedule_format(const elp_schedule* s, char* buf, uint64_t len);
 int elp_schedule_parse(const char* text, elp_schedule* s);
 
+/*
+ * Learned proposal for ring equilibration (no reference counterpart).
+ * A small network (144 int8 features of a site's 5^3 neighbourhood and bond
+ * -> 64 -> 64 -> 11 move logits; bf16 weights on MFMA, fp32 accumulation,
+ * fp32 master parameters) proposes one of the 11 moves of a site; a
+ * Metropolis-Hastings test with the reverse proposal's probability keeps the
+ * uniform measure (the equilibrium of ELP_RULES_BALANCED) stationary.
+ * Parameter vector (14416 floats): W1[144][64] W2[64][64] b2[64] W3[64][16] b3[16].
+ * Moves m of a site P: 0-3 TransForw through common neighbour m (stored
+ * length of P if it has any, else of the partner), 4-7 TransForw moving the
+ * partner's stored length while P holds stored length too, 8 TransBack into P,
+ * 9 TransBack into the partner, 10 DiffuseSL.
+ */
+typedef struct elp_net elp_net;     /* opaque */
+
+int elp_net_create(elp_net** out, int device, uint32_t seed);   /* Glorot-uniform init */
+int elp_net_destroy(elp_net* n);
+int elp_net_params(elp_net* n, float* params, uint64_t count);
+int elp_net_set_params(elp_net* n, const float* params, uint64_t count);
+/* Move probabilities of count feature rows (x: count x 144 int8, prob: count x 11). */
+int elp_net_forward(elp_net* n, const int8_t* x, int count, float* prob);
+/* Gradient of the mean cross-entropy of moves y (count) given features x;
+   grad: 14416 floats, loss: mean loss. */
+int elp_net_gradient(elp_net* n, const int8_t* x, const int32_t* y, int count, float* grad, float* loss);
+/* Append count training samples drawn from the ring engine's lattice: a random
+   site with at least one possible move, and a move drawn with the probability
+   the plain engine (balanced rates) makes it there. */
+int elp_net_harvest(elp_net* n, elp_engine* e, int count, uint32_t seed);
+/* Number of samples, and optionally a copy of them (x: count x 144, y: count). */
+int elp_net_samples(elp_net* n, int* count, int8_t* x, int32_t* y);
+/* Adam (beta 0.9 / 0.999) over the samples in mini-batches; *last_loss = mean
+   loss of the last epoch. */
+int elp_net_train(elp_net* n, int epochs, int batch, float lr, float* last_loss);
+/* n_time time units of the learned-proposal engine on a ring engine with 16^3
+   tiles and the cubic sub-lattice (n_iter iterations of 4 substeps, one
+   proposal per slot per substep); synchronous; counts proposals and
+   accepted moves. */
+int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origins, uint64_t tau0,
+                    uint32_t seed, uint32_t stream, uint64_t* proposals, uint64_t* accepted);
+/* Test entry points: move m at the centre of 5^3 patches (0xFF = outside the
+   tile), and the features of patches. */
+int elp_net_patch_moves(int device, const uint8_t* patches, const int32_t* moves, int count, uint8_t* out,
+                        int32_t* ok, int32_t* rev);
+int elp_net_patch_features(int device, const uint8_t* patches, int count, int8_t* x);
+
 /* Diagnostic (no reference counterpart): device-to-device copy of nbytes
    (multiple of 16) with a plain streaming kernel, reps times; *ms = average
    time per copy.  Used to calibrate HBM byte counters and to measure the

@@ -8,8 +8,10 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <math.h>
 #include "../../include/elp_gpu.h"
 #include "elp_kernels.h"
+#include "elp_learned.h"
 
 namespace {
 
@@ -583,6 +585,300 @@ int elp_set_threads(elp_engine* e, int threads) {
         e->threads = old;
         return fail(ELP_E_ARG, "launch with %d threads is not compiled for this schedule", threads);
     }
+    return ELP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned proposal network
+// ---------------------------------------------------------------------------
+
+struct elp_net {
+    int device;
+    hipStream_t stream = nullptr;
+    float* d_prm = nullptr;                // fp32 master parameters
+    float* d_grad = nullptr;
+    float* d_m1 = nullptr;
+    float* d_m2 = nullptr;
+    float* d_loss = nullptr;
+    unsigned long long* d_stats = nullptr; // proposals, accepted
+    int8_t* d_x = nullptr;                 // training features [cap][144]
+    int32_t* d_y = nullptr;                // training moves [cap]
+    int n_samples = 0, cap = 0;
+    int adam_step = 0;
+};
+
+static void net_free(elp_net* n) {
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    (void)hipFree(n->d_prm); (void)hipFree(n->d_grad); (void)hipFree(n->d_m1); (void)hipFree(n->d_m2);
+    (void)hipFree(n->d_loss); (void)hipFree(n->d_stats); (void)hipFree(n->d_x); (void)hipFree(n->d_y);
+    delete n;
+}
+
+// Glorot-uniform weights from a 64-bit LCG of the seed, zero biases.
+static void net_init_params(std::vector<float>& p, uint32_t seed) {
+    p.assign(elp::LP_NPARAM, 0.f);
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ seed;
+    auto uni = [&x]() {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int fan_in, int fan_out) {
+        const double a = sqrt(6.0 / (fan_in + fan_out));
+        for (int i = 0; i < fan_in * fan_out; i++) p[off + i] = (float)((2.0 * uni() - 1.0) * a);
+    };
+    fill(elp::LP_OFF_W1, elp::LP_NIN, elp::LP_NH);
+    fill(elp::LP_OFF_W2, elp::LP_NH, elp::LP_NH);
+    fill(elp::LP_OFF_W3, elp::LP_NH, elp::LP_NOUT);
+}
+
+extern "C" {
+
+int elp_net_create(elp_net** out, int device, uint32_t seed) {
+    if (!out) return fail(ELP_E_ARG, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ELP_E_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(ELP_E_ARG, "device %d out of range", device);
+    elp_net* n = new elp_net();
+    n->device = device;
+    hipError_t err;
+#define NCHK(call, what)                        \
+    if ((err = (call)) != hipSuccess) {         \
+        net_free(n);                            \
+        return hip_fail(err, what);             \
+    }
+    NCHK(hipSetDevice(device), "hipSetDevice");
+    NCHK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t pb = sizeof(float) * elp::LP_NPARAM;
+    NCHK(hipMalloc(&n->d_prm, pb), "hipMalloc params");
+    NCHK(hipMalloc(&n->d_grad, pb), "hipMalloc grad");
+    NCHK(hipMalloc(&n->d_m1, pb), "hipMalloc adam m");
+    NCHK(hipMalloc(&n->d_m2, pb), "hipMalloc adam v");
+    NCHK(hipMalloc(&n->d_loss, sizeof(float)), "hipMalloc loss");
+    NCHK(hipMalloc(&n->d_stats, 2 * sizeof(unsigned long long)), "hipMalloc stats");
+    std::vector<float> p;
+    net_init_params(p, seed);
+    NCHK(hipMemcpy(n->d_prm, p.data(), pb, hipMemcpyHostToDevice), "upload params");
+#undef NCHK
+    int rc = 0;
+    rc |= elp::lp_launch_fill(n->d_grad, elp::LP_NPARAM, 0.f, n->stream);
+    rc |= elp::lp_launch_fill(n->d_m1, elp::LP_NPARAM, 0.f, n->stream);
+    rc |= elp::lp_launch_fill(n->d_m2, elp::LP_NPARAM, 0.f, n->stream);
+    if (rc || hipStreamSynchronize(n->stream) != hipSuccess) { net_free(n); return fail(ELP_E_HIP, "net init kernels"); }
+    *out = n;
+    return ELP_OK;
+}
+
+int elp_net_destroy(elp_net* n) {
+    if (!n) return ELP_OK;
+    (void)hipSetDevice(n->device);
+    (void)hipStreamSynchronize(n->stream);
+    net_free(n);
+    return ELP_OK;
+}
+
+int elp_net_params(elp_net* n, float* params, uint64_t count) {
+    if (!n || !params || count != (uint64_t)elp::LP_NPARAM) return fail(ELP_E_ARG, "need %d parameters", elp::LP_NPARAM);
+    HIPCHK(hipSetDevice(n->device));
+    HIPCHK(hipMemcpyAsync(params, n->d_prm, sizeof(float) * count, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return ELP_OK;
+}
+
+int elp_net_set_params(elp_net* n, const float* params, uint64_t count) {
+    if (!n || !params || count != (uint64_t)elp::LP_NPARAM) return fail(ELP_E_ARG, "need %d parameters", elp::LP_NPARAM);
+    HIPCHK(hipSetDevice(n->device));
+    HIPCHK(hipMemcpyAsync(n->d_prm, params, sizeof(float) * count, hipMemcpyHostToDevice, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return ELP_OK;
+}
+
+// device scratch of a host call, freed on return
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+};
+
+int elp_net_forward(elp_net* n, const int8_t* x, int count, float* prob) {
+    if (!n || !x || !prob || count <= 0) return fail(ELP_E_ARG, "bad forward arguments");
+    HIPCHK(hipSetDevice(n->device));
+    DevBuf dx, dp;
+    HIPCHK(hipMalloc(&dx.p, (size_t)count * elp::LP_NIN));
+    HIPCHK(hipMalloc(&dp.p, sizeof(float) * (size_t)count * elp::LP_NMOVE));
+    HIPCHK(hipMemcpyAsync(dx.p, x, (size_t)count * elp::LP_NIN, hipMemcpyHostToDevice, n->stream));
+    int rc = elp::lp_launch_forward(n->d_prm, (const int8_t*)dx.p, count, (float*)dp.p, n->stream);
+    if (rc) return fail(ELP_E_HIP, "forward kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(prob, dp.p, sizeof(float) * (size_t)count * elp::LP_NMOVE, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return ELP_OK;
+}
+
+int elp_net_gradient(elp_net* n, const int8_t* x, const int32_t* y, int count, float* grad, float* loss) {
+    if (!n || !x || !y || !grad || count <= 0) return fail(ELP_E_ARG, "bad gradient arguments");
+    for (int i = 0; i < count; i++)
+        if (y[i] < 0 || y[i] >= elp::LP_NMOVE) return fail(ELP_E_ARG, "label %d out of range", y[i]);
+    HIPCHK(hipSetDevice(n->device));
+    DevBuf dx, dy, dg;
+    HIPCHK(hipMalloc(&dx.p, (size_t)count * elp::LP_NIN));
+    HIPCHK(hipMalloc(&dy.p, sizeof(int32_t) * count));
+    HIPCHK(hipMalloc(&dg.p, sizeof(float) * elp::LP_NPARAM));
+    HIPCHK(hipMemcpyAsync(dx.p, x, (size_t)count * elp::LP_NIN, hipMemcpyHostToDevice, n->stream));
+    HIPCHK(hipMemcpyAsync(dy.p, y, sizeof(int32_t) * count, hipMemcpyHostToDevice, n->stream));
+    int rc = elp::lp_launch_fill((float*)dg.p, elp::LP_NPARAM, 0.f, n->stream);
+    rc = rc ? rc : elp::lp_launch_fill(n->d_loss, 1, 0.f, n->stream);
+    rc = rc ? rc : elp::lp_launch_grad(n->d_prm, (const int8_t*)dx.p, (const int32_t*)dy.p, count, 1.f / count,
+                                       (float*)dg.p, n->d_loss, n->stream);
+    if (rc) return fail(ELP_E_HIP, "gradient kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(grad, dg.p, sizeof(float) * elp::LP_NPARAM, hipMemcpyDeviceToHost, n->stream));
+    float l = 0;
+    HIPCHK(hipMemcpyAsync(&l, n->d_loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    if (loss) *loss = l / count;
+    return ELP_OK;
+}
+
+int elp_net_harvest(elp_net* n, elp_engine* e, int count, uint32_t seed) {
+    if (!n || !e || count <= 0) return fail(ELP_E_ARG, "bad harvest arguments");
+    if (e->kind != 0) return fail(ELP_E_ARG, "harvest needs a ring engine");
+    if (!e->uploaded) return fail(ELP_E_STATE, "harvest before upload");
+    if (e->device != n->device) return fail(ELP_E_ARG, "engine and network on different devices");
+    HIPCHK(hipSetDevice(n->device));
+    HIPCHK(hipStreamSynchronize(e->stream));           // the lattice of the engine's last run
+    if (n->n_samples + count > n->cap) {
+        int cap = n->cap ? n->cap : 1 << 16;
+        while (cap < n->n_samples + count) cap *= 2;
+        int8_t* x = nullptr;
+        int32_t* y = nullptr;
+        HIPCHK(hipMalloc(&x, (size_t)cap * elp::LP_NIN));
+        hipError_t err = hipMalloc(&y, sizeof(int32_t) * (size_t)cap);
+        if (err != hipSuccess) { (void)hipFree(x); return hip_fail(err, "hipMalloc samples"); }
+        if (n->n_samples) {
+            HIPCHK(hipMemcpyAsync(x, n->d_x, (size_t)n->n_samples * elp::LP_NIN, hipMemcpyDeviceToDevice, n->stream));
+            HIPCHK(hipMemcpyAsync(y, n->d_y, sizeof(int32_t) * n->n_samples, hipMemcpyDeviceToDevice, n->stream));
+            HIPCHK(hipStreamSynchronize(n->stream));
+        }
+        (void)hipFree(n->d_x);
+        (void)hipFree(n->d_y);
+        n->d_x = x;
+        n->d_y = y;
+        n->cap = cap;
+    }
+    int rc = elp::lp_launch_harvest((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, count, seed, (uint32_t)n->n_samples,
+                                    n->d_x + (size_t)n->n_samples * elp::LP_NIN, n->d_y + n->n_samples, n->stream);
+    if (rc) return fail(ELP_E_HIP, "harvest kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    n->n_samples += count;
+    return ELP_OK;
+}
+
+int elp_net_samples(elp_net* n, int* count, int8_t* x, int32_t* y) {
+    if (!n || !count) return fail(ELP_E_ARG, "NULL argument");
+    *count = n->n_samples;
+    HIPCHK(hipSetDevice(n->device));
+    if (x && n->n_samples)
+        HIPCHK(hipMemcpyAsync(x, n->d_x, (size_t)n->n_samples * elp::LP_NIN, hipMemcpyDeviceToHost, n->stream));
+    if (y && n->n_samples)
+        HIPCHK(hipMemcpyAsync(y, n->d_y, sizeof(int32_t) * n->n_samples, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return ELP_OK;
+}
+
+int elp_net_train(elp_net* n, int epochs, int batch, float lr, float* last_loss) {
+    if (!n || epochs <= 0 || batch <= 0 || !(lr > 0)) return fail(ELP_E_ARG, "bad training arguments");
+    if (n->n_samples == 0) return fail(ELP_E_STATE, "no training samples (elp_net_harvest)");
+    HIPCHK(hipSetDevice(n->device));
+    float l = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        int rc = elp::lp_launch_fill(n->d_loss, 1, 0.f, n->stream);
+        for (int b0 = 0; b0 < n->n_samples && !rc; b0 += batch) {
+            const int nb = (n->n_samples - b0) < batch ? (n->n_samples - b0) : batch;
+            rc = elp::lp_launch_grad(n->d_prm, n->d_x + (size_t)b0 * elp::LP_NIN, n->d_y + b0, nb, 1.f / nb, n->d_grad,
+                                     n->d_loss, n->stream);
+            if (!rc)
+                rc = elp::lp_launch_adam(n->d_prm, n->d_grad, n->d_m1, n->d_m2, elp::LP_NPARAM, lr, 0.9f, 0.999f, 1e-8f,
+                                         ++n->adam_step, n->stream);
+        }
+        if (rc) return fail(ELP_E_HIP, "training kernels: %s", hipGetErrorString((hipError_t)rc));
+        HIPCHK(hipMemcpyAsync(&l, n->d_loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+        HIPCHK(hipStreamSynchronize(n->stream));
+    }
+    if (last_loss) *last_loss = l / n->n_samples;
+    return ELP_OK;
+}
+
+int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origins, uint64_t tau0, uint32_t seed,
+                    uint32_t stream, uint64_t* proposals, uint64_t* accepted) {
+    if (!e || !n) return fail(ELP_E_ARG, "NULL engine or network");
+    if (e->kind != 0 || e->tt != 16 || e->tu != 16 || e->tv != 16 || e->sublattice != ELP_SUBLAT_CUBIC)
+        return fail(ELP_E_ARG, "the learned proposal runs on ring engines with 16^3 tiles and the cubic sub-lattice");
+    if (e->device != n->device) return fail(ELP_E_ARG, "engine and network on different devices");
+    if (n_time < 0 || (n_time > 0 && !origins)) return fail(ELP_E_ARG, "bad n_time/origins");
+    if (!e->uploaded) return fail(ELP_E_STATE, "run before upload");
+    for (int i = 0; i < n_time; i++) {
+        const int32_t* o = origins + 3 * i;
+        if (o[0] < 0 || o[0] >= 16 || o[1] < 0 || o[1] >= 16 || o[2] < 0 || o[2] >= 16)
+            return fail(ELP_E_ARG, "origin (%d,%d,%d) of time unit %d outside the tile", o[0], o[1], o[2], i);
+    }
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(n->stream));           // parameters from the last training step
+    unsigned long long zero[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(n->d_stats, zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
+    elp::RingLaunch p{};
+    p.LT = e->LT; p.LU = e->LU; p.LV = e->LV;
+    p.ntile_t = e->LT / 16; p.ntile_u = e->LU / 16;
+    p.seed = seed; p.stream = stream;
+    p.n_iter = e->n_iter;
+    const int nblocks = (e->LT / 16) * (e->LU / 16) * (e->LV / 16);
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    for (int i = 0; i < n_time; i++) {
+        p.ot = origins[3 * i]; p.ou = origins[3 * i + 1]; p.ov = origins[3 * i + 2];
+        p.tau = (uint32_t)(tau0 + (uint64_t)i);
+        int rc = elp::lp_launch_mc((uint8_t*)e->d_lat, p, nblocks, n->d_prm, n->d_stats, e->stream);
+        if (rc) return fail(ELP_E_HIP, "learned kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+    }
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    e->timed = true;
+    unsigned long long st[2];
+    HIPCHK(hipMemcpyAsync(st, n->d_stats, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (proposals) *proposals = st[0];
+    if (accepted) *accepted = st[1];
+    return ELP_OK;
+}
+
+int elp_net_patch_moves(int device, const uint8_t* patches, const int32_t* moves, int count, uint8_t* out,
+                        int32_t* ok, int32_t* rev) {
+    if (!patches || !moves || !out || !ok || !rev || count <= 0) return fail(ELP_E_ARG, "bad patch-move arguments");
+    HIPCHK(hipSetDevice(device));
+    DevBuf dp, dm, dout, dok, drev;
+    HIPCHK(hipMalloc(&dp.p, (size_t)count * 125));
+    HIPCHK(hipMalloc(&dm.p, sizeof(int32_t) * count));
+    HIPCHK(hipMalloc(&dout.p, (size_t)count * 125));
+    HIPCHK(hipMalloc(&dok.p, sizeof(int32_t) * count));
+    HIPCHK(hipMalloc(&drev.p, sizeof(int32_t) * count));
+    HIPCHK(hipMemcpy(dp.p, patches, (size_t)count * 125, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dm.p, moves, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    int rc = elp::lp_launch_patch_move((const uint8_t*)dp.p, (const int32_t*)dm.p, count, (uint8_t*)dout.p,
+                                       (int32_t*)dok.p, (int32_t*)drev.p, 0);
+    if (rc) return fail(ELP_E_HIP, "patch-move kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpy(out, dout.p, (size_t)count * 125, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ok, dok.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rev, drev.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost));
+    return ELP_OK;
+}
+
+int elp_net_patch_features(int device, const uint8_t* patches, int count, int8_t* x) {
+    if (!patches || !x || count <= 0) return fail(ELP_E_ARG, "bad patch-feature arguments");
+    HIPCHK(hipSetDevice(device));
+    DevBuf dp, dx;
+    HIPCHK(hipMalloc(&dp.p, (size_t)count * 125));
+    HIPCHK(hipMalloc(&dx.p, (size_t)count * elp::LP_NIN));
+    HIPCHK(hipMemcpy(dp.p, patches, (size_t)count * 125, hipMemcpyHostToDevice));
+    int rc = elp::lp_launch_patch_features((const uint8_t*)dp.p, count, (int8_t*)dx.p, 0);
+    if (rc) return fail(ELP_E_HIP, "patch-feature kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpy(x, dx.p, (size_t)count * elp::LP_NIN, hipMemcpyDeviceToHost));
     return ELP_OK;
 }
 

@@ -1,0 +1,705 @@
+// elp_learned.hip -- learned move proposal for ring equilibration (no
+// reference counterpart; it biases which move the ring engine tries and is
+// corrected by Metropolis-Hastings so that the uniform measure -- the
+// equilibrium of the balanced ring rates -- stays stationary).
+//
+// Network (14416 parameters, fp32 master copy; every product runs on
+// gfx950 MFMA with bf16 operands and fp32 accumulation):
+//   x  = 144 features of a site, int8: 125 occupancy bits of the 5^3 patch
+//        around the site, one-hot of its bond code (16), its stored-length
+//        bit, the stored-length bit of its bond partner, and a constant 1
+//        (the bias of layer 1);
+//   h1 = relu(x W1)            W1: 144 x 64
+//   h2 = relu(h1 W2 + b2)      W2: 64 x 64
+//   z  = h2 W3 + b3            W3: 64 x 16, logits of the 11 moves of a site
+//   q  = softmax(z[0..10])
+// Moves of the site P (all change only P and sites one step away):
+//   m 0..3   TransForw through common neighbour k = m (k bit 0: decomposition,
+//            bit 1: order, as the r bits of the plain move); the monomer that
+//            moves is the stored-length one of P if P holds stored length,
+//            else that of the bond partner
+//   m 4..7   TransForw k = m-4 moving the bond partner's stored length while
+//            P holds stored length too (both ends)
+//   m 8      TransBack folding the next monomer into P (P gains stored length)
+//   m 9      TransBack folding it into the partner after it
+//   m 10     DiffuseSL across the bond
+// Each (state, move) gives a distinct new state and its reverse is a move
+// of the same site, so q(m | S) and q(rev | S') give the exact MH ratio.
+// Concurrent sites use the cubic sub-lattice (pitch 4): a patch (radius 2)
+// never meets another site's moves (radius 1), so the sites of a substep
+// are independent.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "elp_device.h"
+#include "elp_learned.h"
+
+namespace elp {
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);          // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- patch moves
+// Patch: 5 x 5 x 5 site bytes, index t + 5u + 25v, the site at the centre.
+// 0xFF marks a site outside the tile: a move touching it is invalid.
+constexpr int PC = 62;
+constexpr uint8_t WALL = 0xFF;
+
+__device__ __forceinline__ int pstep(int i, uint32_t b) { return i + bdt(b) + 5 * bdu(b) + 25 * bdv(b); }
+
+// Apply move m at the centre of the patch q.  Returns 1 and the reverse move
+// index if the move is possible (q then holds the new state; ch[0..2] = the
+// patch indices written), else 0 and q is unchanged.
+__device__ int lp_move(uint8_t* q, int m, int* rev, int* ch) {
+    const uint32_t s = q[PC];
+    if (!s || s == WALL) return 0;
+    const uint32_t next = s & 0xfu, label = s & 0x30u, sl = s & 0x40u;
+    const int in = pstep(PC, next);
+    uint32_t ns = q[in];
+    if (ns == WALL) return 0;
+    const uint32_t nsl = ns & 0x40u;
+    if (m < 8) {                                       // TransForw
+        const uint32_t k = (uint32_t)m & 3u;
+        if (!(sl | nsl)) return 0;
+        bool moveFirst;
+        if (m < 4) moveFirst = sl != 0;                // P's stored length if any, else the partner's
+        else { if (!(sl && nsl)) return 0; moveFirst = false; }
+        uint32_t nb1 = trans_first(next, k & 1u);
+        uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
+        if (!(k & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+        const int id = pstep(PC, nb1);
+        if (q[id]) return 0;                           // occupied or outside
+        uint32_t d = nb2, p = s;
+        if (moveFirst) {
+            p = nb1 | ((label >> 1) & 0x10u);
+            d |= label & 0x10u;
+        } else {
+            p = nb1 | label | sl;
+            d |= (ns & 0x20u) >> 1;
+            ns &= 0x1fu;
+        }
+        q[PC] = (uint8_t)p;
+        q[id] = (uint8_t)d;
+        q[in] = (uint8_t)ns;
+        *rev = moveFirst ? 8 : 9;
+        ch[0] = PC; ch[1] = id; ch[2] = in;
+        return 1;
+    }
+    if (m < 10) {                                      // TransBack
+        if (nsl) return 0;                             // the folded monomer must be alone
+        const uint32_t srcNext = ns & 0xfu;
+        if ((next | srcNext) != 0xfu && (next & srcNext)) return 0;
+        const uint32_t newNext = bond_add(next, srcNext);
+        if (newNext == 0u || newNext == 0x3u || newNext == 0xcu) return 0;
+        const int iw = pstep(PC, newNext);
+        uint32_t ws = q[iw];
+        if (ws == WALL) return 0;
+        const uint32_t wsl = ws & 0x40u;
+        const bool intoP = m == 8;
+        if (intoP ? sl != 0 : wsl != 0) return 0;
+        const uint32_t srcLabel = ns & 0x30u;
+        uint32_t p;
+        if (intoP) {
+            p = newNext | (label << 1) | srcLabel | 0x40u;
+        } else {
+            p = newNext | label | sl;
+            ws = (ws & 0x3fu) | (srcLabel << 1) | 0x40u;
+        }
+        // reverse: TransForw through the vacated site; k: its position
+        // among the four common neighbours of P and the new partner
+        int k = 0;
+        for (; k < 4; k++) {
+            uint32_t nb1 = trans_first(newNext, (uint32_t)k & 1u);
+            const uint32_t nb2 = bond_add((~nb1) & 0xfu, newNext);
+            if (!(k & 2)) nb1 = nb2;
+            if (nb1 == next) break;
+        }
+        q[in] = 0;
+        q[PC] = (uint8_t)p;
+        q[iw] = (uint8_t)ws;
+        // the partner's stored length moves back: forced if P has none
+        *rev = intoP ? k : ((sl != 0) ? k + 4 : k);
+        ch[0] = PC; ch[1] = in; ch[2] = iw;
+        return 1;
+    }
+    if (m == 10) {                                     // DiffuseSL
+        if ((nsl + sl) != 0x40u) return 0;
+        uint32_t p;
+        if (sl) {
+            ns = ns | ((label & 0x10u) << 1) | 0x40u;
+            p = next | ((label >> 1) & 0x10u);
+        } else {
+            p = next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u;
+            ns &= 0x1fu;
+        }
+        q[PC] = (uint8_t)p;
+        q[in] = (uint8_t)ns;
+        *rev = 10;
+        ch[0] = PC; ch[1] = in; ch[2] = in;
+        return 1;
+    }
+    return 0;
+}
+
+constexpr uint16_t BF_ONE = 0x3F80u;                  // bf16 1.0
+
+// Features 125..143 of the centre site of a patch: bond one-hot, stored
+// length of the site and of its bond partner, constant 1.
+__device__ void lp_features_tail(const uint8_t* q, uint16_t* x) {
+    const uint16_t one = BF_ONE;
+    const uint32_t s = q[PC];
+    const uint32_t b = (s == WALL) ? 0u : (s & 0xfu);
+    for (int i = 0; i < 16; i++) x[125 + i] = (i == (int)b && s && s != WALL) ? one : 0;
+    const int in = pstep(PC, b);
+    x[141] = (s != WALL && (s & 0x40u)) ? one : 0;
+    x[142] = (s && s != WALL && q[in] != WALL && (q[in] & 0x40u)) ? one : 0;
+    x[143] = one;
+}
+
+// Features of the centre site of a patch (row of 144 bf16 values 0 / 1):
+// occupancy of the 125 patch sites, then the tail above.
+__device__ void lp_features(const uint8_t* q, uint16_t* x) {
+    for (int i = 0; i < 125; i++) x[i] = q[i] ? BF_ONE : 0;
+    lp_features_tail(q, x);
+}
+
+// ---------------------------------------------------------------- MFMA tiles
+// One wave, 16 x 16 output tile: D(r, c) = sum_k A(r, k) B(k, c), K a
+// multiple of 16; A(r, k) = a[r*ar + k*ak], B(k, c) = b[k*bk + c*bc] (bf16).
+// v_mfma_f32_16x16x16_bf16: lane l holds A(l%16, 4(l/16)+j), B(4(l/16)+j, l%16),
+// j < 4, and D(4(l/16)+i, l%16), i < 4.
+__device__ __forceinline__ f32x4 mfma_tile(const uint16_t* a, int ar, int ak, const uint16_t* b, int bk, int bc, int K,
+                                           f32x4 acc) {
+    const int lane = threadIdx.x & 63;
+    const int rc = lane & 15, kq = 4 * (lane >> 4);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            av[j] = (short)a[rc * ar + (k0 + kq + j) * ak];
+            bv[j] = (short)b[(k0 + kq + j) * bk + rc * bc];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Shared-memory image of the network and one batch of 64 sites.
+struct LpShared {
+    uint16_t W1T[LP_NH][LP_NIN];     // [out][in]
+    uint16_t W2T[LP_NH][LP_NH];
+    uint16_t W3T[LP_NOUT][LP_NH];
+    float b2[LP_NH], b3[LP_NOUT];
+    uint16_t X[64][LP_NIN];
+    uint16_t H1[64][LP_NH];
+    uint16_t H2[64][LP_NH];
+    float Z[64][LP_NOUT];
+};
+
+__device__ void lp_load_net(LpShared& S, const float* __restrict__ prm) {
+    for (int i = threadIdx.x; i < LP_NIN * LP_NH; i += blockDim.x) {
+        const int in = i / LP_NH, out = i % LP_NH;
+        S.W1T[out][in] = to_bf16(prm[LP_OFF_W1 + i]);
+    }
+    for (int i = threadIdx.x; i < LP_NH * LP_NH; i += blockDim.x) {
+        const int in = i / LP_NH, out = i % LP_NH;
+        S.W2T[out][in] = to_bf16(prm[LP_OFF_W2 + i]);
+    }
+    for (int i = threadIdx.x; i < LP_NH * LP_NOUT; i += blockDim.x) {
+        const int in = i / LP_NOUT, out = i % LP_NOUT;
+        S.W3T[out][in] = to_bf16(prm[LP_OFF_W3 + i]);
+    }
+    for (int i = threadIdx.x; i < LP_NH; i += blockDim.x) S.b2[i] = prm[LP_OFF_B2 + i];
+    for (int i = threadIdx.x; i < LP_NOUT; i += blockDim.x) S.b3[i] = prm[LP_OFF_B3 + i];
+}
+
+// Forward pass of the 64 rows of S.X: H1, H2 (bf16) and logits Z (fp32).
+// Called by all threads of the workgroup (blockDim = 256, 4 waves).
+__device__ void lp_forward(LpShared& S) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, r4 = 4 * (lane >> 4);
+    for (int t = wave; t < 16; t += 4) {               // layer 1: 4 x 4 tiles
+        const int rb = t >> 2, cb = t & 3;
+        f32x4 acc = {0, 0, 0, 0};
+        acc = mfma_tile(&S.X[rb * 16][0], LP_NIN, 1, &S.W1T[cb * 16][0], 1, LP_NIN, LP_NIN, acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) S.H1[rb * 16 + r4 + i][cb * 16 + col] = to_bf16(fmaxf(acc[i], 0.f));
+    }
+    __syncthreads();
+    for (int t = wave; t < 16; t += 4) {               // layer 2
+        const int rb = t >> 2, cb = t & 3;
+        f32x4 acc = {0, 0, 0, 0};
+        acc = mfma_tile(&S.H1[rb * 16][0], LP_NH, 1, &S.W2T[cb * 16][0], 1, LP_NH, LP_NH, acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            S.H2[rb * 16 + r4 + i][cb * 16 + col] = to_bf16(fmaxf(acc[i] + S.b2[cb * 16 + col], 0.f));
+    }
+    __syncthreads();
+    {                                                  // layer 3: one tile per wave
+        const int rb = wave;
+        f32x4 acc = {0, 0, 0, 0};
+        acc = mfma_tile(&S.H2[rb * 16][0], LP_NH, 1, &S.W3T[0][0], 1, LP_NH, LP_NH, acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) S.Z[rb * 16 + r4 + i][col] = acc[i] + S.b3[col];
+    }
+    __syncthreads();
+}
+
+// softmax over the 11 moves of row b (fp32)
+__device__ void lp_softmax(const float* z, float* p) {
+    float mx = z[0];
+    for (int m = 1; m < LP_NMOVE; m++) mx = fmaxf(mx, z[m]);
+    float sum = 0;
+    for (int m = 0; m < LP_NMOVE; m++) { p[m] = __expf(z[m] - mx); sum += p[m]; }
+    const float inv = 1.f / sum;
+    for (int m = 0; m < LP_NMOVE; m++) p[m] *= inv;
+}
+
+// ---------------------------------------------------------------- kernels
+// Probabilities of n feature rows (tests and diagnostics).
+__global__ __launch_bounds__(256) void lp_forward_kernel(const float* __restrict__ prm, const int8_t* __restrict__ x,
+                                                          int n, float* __restrict__ prob) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    LpShared& S = *reinterpret_cast<LpShared*>(lds_raw);
+    lp_load_net(S, prm);
+    for (int b0 = blockIdx.x * 64; b0 < n; b0 += gridDim.x * 64) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 64 * LP_NIN; i += blockDim.x) {
+            const int r = i / LP_NIN, c = i % LP_NIN;
+            S.X[r][c] = (b0 + r < n) ? to_bf16((float)x[(size_t)(b0 + r) * LP_NIN + c]) : 0;
+        }
+        __syncthreads();
+        lp_forward(S);
+        if (threadIdx.x < 64 && b0 + threadIdx.x < n) {
+            float p[LP_NMOVE];
+            lp_softmax(S.Z[threadIdx.x], p);
+            for (int m = 0; m < LP_NMOVE; m++) prob[(size_t)(b0 + threadIdx.x) * LP_NMOVE + m] = p[m];
+        }
+    }
+}
+
+// Gradient of scale * sum of cross-entropy losses over n samples, accumulated
+// (atomically) into grad[LP_NPARAM]; loss sum into *loss.  Each wave keeps
+// its weight-gradient tiles in registers over all batches of its workgroup.
+struct LpTrainShared {
+    LpShared F;
+    uint16_t D3[64][LP_NOUT];
+    uint16_t D2[64][LP_NH];
+    uint16_t D1[64][LP_NH];
+    float d3f[64][LP_NOUT];
+    int y[64];
+};
+
+__global__ __launch_bounds__(256) void lp_grad_kernel(const float* __restrict__ prm, const int8_t* __restrict__ x,
+                                                       const int32_t* __restrict__ y, int n, float scale,
+                                                       float* __restrict__ grad, float* __restrict__ loss) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    LpTrainShared& T = *reinterpret_cast<LpTrainShared*>(lds_raw);
+    LpShared& S = T.F;
+    lp_load_net(S, prm);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, r4 = 4 * (lane >> 4);
+    f32x4 gW1[9], gW2[4], gW3;
+    for (int i = 0; i < 9; i++) gW1[i] = f32x4{0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) gW2[i] = f32x4{0, 0, 0, 0};
+    gW3 = f32x4{0, 0, 0, 0};
+    float gb2 = 0, gb3 = 0, lsum = 0;
+    for (int b0 = blockIdx.x * 64; b0 < n; b0 += gridDim.x * 64) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 64 * LP_NIN; i += blockDim.x) {
+            const int r = i / LP_NIN, c = i % LP_NIN;
+            S.X[r][c] = (b0 + r < n) ? to_bf16((float)x[(size_t)(b0 + r) * LP_NIN + c]) : 0;
+        }
+        if (threadIdx.x < 64) T.y[threadIdx.x] = (b0 + threadIdx.x < n) ? y[b0 + threadIdx.x] : -1;
+        __syncthreads();
+        lp_forward(S);
+        if (threadIdx.x < 64) {                        // dL/dz = scale * (softmax - onehot); padded rows 0
+            const int b = threadIdx.x;
+            float p[LP_NMOVE];
+            lp_softmax(S.Z[b], p);
+            const int yb = T.y[b];
+            for (int m = 0; m < LP_NOUT; m++) {
+                float d = 0;
+                if (yb >= 0 && m < LP_NMOVE) d = scale * (p[m] - (m == yb ? 1.f : 0.f));
+                T.d3f[b][m] = d;
+                T.D3[b][m] = to_bf16(d);
+            }
+            if (yb >= 0) lsum += -__logf(fmaxf(p[yb], 1e-30f));
+        }
+        __syncthreads();
+        // gW3 (64 x 16) += H2^T D3: row block = wave
+        gW3 = mfma_tile(&S.H2[0][wave * 16], 1, LP_NH, &T.D3[0][0], LP_NOUT, 1, 64, gW3);
+        // D2 = (D3 W3^T) * [H2 > 0]
+        for (int t = wave; t < 16; t += 4) {
+            const int rb = t >> 2, cb = t & 3;
+            f32x4 acc = {0, 0, 0, 0};
+            acc = mfma_tile(&T.D3[rb * 16][0], LP_NOUT, 1, &S.W3T[0][cb * 16], LP_NH, 1, LP_NOUT, acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = rb * 16 + r4 + i, c = cb * 16 + col;
+                T.D2[r][c] = S.H2[r][c] ? to_bf16(acc[i]) : 0;
+            }
+        }
+        if (threadIdx.x < LP_NOUT) {
+            float s = 0;
+            for (int b = 0; b < 64; b++) s += T.d3f[b][threadIdx.x];
+            gb3 += s;
+        }
+        __syncthreads();
+        // gW2 (64 x 64) += H1^T D2: tiles (rb, cb) with rb = wave
+        for (int cb = 0; cb < 4; cb++)
+            gW2[cb] = mfma_tile(&S.H1[0][wave * 16], 1, LP_NH, &T.D2[0][cb * 16], LP_NH, 1, 64, gW2[cb]);
+        // D1 = (D2 W2^T) * [H1 > 0]
+        for (int t = wave; t < 16; t += 4) {
+            const int rb = t >> 2, cb = t & 3;
+            f32x4 acc = {0, 0, 0, 0};
+            acc = mfma_tile(&T.D2[rb * 16][0], LP_NH, 1, &S.W2T[0][cb * 16], LP_NH, 1, LP_NH, acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = rb * 16 + r4 + i, c = cb * 16 + col;
+                T.D1[r][c] = S.H1[r][c] ? to_bf16(acc[i]) : 0;
+            }
+        }
+        if (threadIdx.x < LP_NH) {
+            float s = 0;
+            for (int b = 0; b < 64; b++) s += from_bf16(T.D2[b][threadIdx.x]);
+            gb2 += s;
+        }
+        __syncthreads();
+        // gW1 (144 x 64) += X^T D1: 9 x 4 tiles, wave owns row blocks wave, wave+4, wave+8 (< 9)
+        for (int j = 0; j < 9; j++) {
+            const int t = wave + 4 * j;
+            if (t >= 36) break;
+            const int rb = t >> 2, cb = t & 3;
+            gW1[j] = mfma_tile(&S.X[0][rb * 16], 1, LP_NIN, &T.D1[0][cb * 16], LP_NH, 1, 64, gW1[j]);
+        }
+    }
+    // flush
+    for (int j = 0; j < 9; j++) {
+        const int t = wave + 4 * j;
+        if (t >= 36) break;
+        const int rb = t >> 2, cb = t & 3;
+        for (int i = 0; i < 4; i++)
+            atomicAdd(&grad[LP_OFF_W1 + (rb * 16 + r4 + i) * LP_NH + cb * 16 + col], gW1[j][i]);
+    }
+    for (int cb = 0; cb < 4; cb++)
+        for (int i = 0; i < 4; i++)
+            atomicAdd(&grad[LP_OFF_W2 + (wave * 16 + r4 + i) * LP_NH + cb * 16 + col], gW2[cb][i]);
+    for (int i = 0; i < 4; i++) atomicAdd(&grad[LP_OFF_W3 + (wave * 16 + r4 + i) * LP_NOUT + col], gW3[i]);
+    if (threadIdx.x < LP_NH) atomicAdd(&grad[LP_OFF_B2 + threadIdx.x], gb2);
+    if (threadIdx.x < LP_NOUT) atomicAdd(&grad[LP_OFF_B3 + threadIdx.x], gb3);
+    if (threadIdx.x < 64) atomicAdd(loss, lsum);
+}
+
+// Adam step on the fp32 master parameters; clears the gradient.
+__global__ void lp_adam_kernel(float* __restrict__ prm, float* __restrict__ grad, float* __restrict__ m1,
+                               float* __restrict__ m2, int n, float lr, float beta1, float beta2, float eps, float c1,
+                               float c2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float g = grad[i];
+        const float a = beta1 * m1[i] + (1.f - beta1) * g;
+        const float b = beta2 * m2[i] + (1.f - beta2) * g * g;
+        m1[i] = a;
+        m2[i] = b;
+        prm[i] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
+        grad[i] = 0.f;
+    }
+}
+
+__global__ void lp_fill_kernel(float* __restrict__ p, int n, float v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Patch of site (t,u,v) of a periodic ring lattice (row-major, t fastest).
+__device__ void gather_periodic(const uint8_t* __restrict__ lat, int LT, int LU, int LV, int t, int u, int v,
+                                uint8_t* q) {
+    for (int i = 0; i < 125; i++) {
+        int a = t + i % 5 - 2, b = u + (i / 5) % 5 - 2, c = v + i / 25 - 2;
+        a += (a < 0) ? LT : 0; a -= (a >= LT) ? LT : 0;
+        b += (b < 0) ? LU : 0; b -= (b >= LU) ? LU : 0;
+        c += (c < 0) ? LV : 0; c -= (c >= LV) ? LV : 0;
+        q[i] = lat[(size_t)a + (size_t)LT * ((size_t)b + (size_t)LU * (size_t)c)];
+    }
+}
+
+// Training samples from a ring lattice: per sample a uniformly drawn site
+// with at least one possible move (up to 32 draws), and a move drawn from
+// the probabilities with which the plain engine under balanced rates makes
+// each move there (TransForw 1/2 over 4 neighbours x the stored-length
+// choice, DiffuseSL 1/4, TransBack 1/4 with the forced fold at 1/4).
+__global__ __launch_bounds__(64) void lp_harvest_kernel(const uint8_t* __restrict__ lat, int LT, int LU, int LV, int n,
+                                                         uint32_t seed, uint32_t stream, int8_t* __restrict__ xo,
+                                                         int32_t* __restrict__ yo) {
+    __shared__ uint8_t qs[64][128];
+    __shared__ uint8_t ws[64][128];
+    __shared__ uint16_t xs[64][LP_NIN];
+    const int tid = threadIdx.x;
+    uint8_t* q = qs[tid];
+    uint8_t* w = ws[tid];
+    for (int i0 = blockIdx.x * 64; i0 < n; i0 += gridDim.x * 64) {
+        const int i = i0 + tid;
+        if (i >= n) continue;
+        int label = -1;                                // no site with a possible move: skipped in training
+        for (int tr = 0; tr < 32; tr++) {
+            const uint4 r = philox4x32_10(make_uint4((uint32_t)i, (uint32_t)tr, 0xA5A5A5A5u, 0u), seed, stream);
+            const uint64_t site = (((uint64_t)r.x << 32) | r.y) % ((uint64_t)LT * LU * LV);
+            const int t = (int)(site % LT), u = (int)((site / LT) % LU), v = (int)(site / ((uint64_t)LT * LU));
+            gather_periodic(lat, LT, LU, LV, t, u, v, q);
+            float wt[LP_NMOVE], tot = 0;
+            const uint32_t s = q[PC];
+            bool bothSL = false, forcedTB = false;
+            if (s && s != WALL) {
+                const uint32_t nx = s & 0xfu, ns = q[pstep(PC, nx)];
+                bothSL = (s & 0x40u) && ns != WALL && (ns & 0x40u);
+                const uint32_t nn = bond_add(nx, ns & 0xfu);
+                const uint32_t ws = q[pstep(PC, nn)];
+                forcedTB = ((s & 0x40u) != 0) != (ws != WALL && (ws & 0x40u));
+            }
+            for (int m = 0; m < LP_NMOVE; m++) {
+                for (int k = 0; k < 125; k++) w[k] = q[k];
+                int rv, ch[3];
+                wt[m] = 0;
+                if (!lp_move(w, m, &rv, ch)) continue;
+                if (m < 8) wt[m] = bothSL ? 1.f / 16 : 1.f / 8;
+                else if (m < 10) wt[m] = forcedTB ? 1.f / 16 : 1.f / 8;
+                else wt[m] = 0.25f;
+                tot += wt[m];
+            }
+            if (tot > 0) {
+                float u01 = (float)(r.z >> 8) * (1.f / 16777216.f) * tot;
+                label = LP_NMOVE - 1;
+                for (int m = 0; m < LP_NMOVE; m++) {
+                    if (wt[m] > 0 && u01 < wt[m]) { label = m; break; }
+                    u01 -= wt[m];
+                }
+                while (wt[label] == 0) label--;
+                break;
+            }
+        }
+        lp_features(q, xs[tid]);
+        for (int c = 0; c < LP_NIN; c++) xo[(size_t)i * LP_NIN + c] = xs[tid][c] ? 1 : 0;
+        yo[i] = label;
+    }
+}
+
+// One move m at the centre of each of n patches (tests): out = patch after
+// the move (unchanged if impossible), ok[i], rev[i].
+__global__ void lp_patch_move_kernel(const uint8_t* __restrict__ in, const int32_t* __restrict__ mv, int n,
+                                     uint8_t* __restrict__ out, int32_t* __restrict__ ok, int32_t* __restrict__ rev) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t q[125];
+    for (int k = 0; k < 125; k++) q[k] = in[(size_t)i * 125 + k];
+    int r = -1, ch[3];
+    ok[i] = lp_move(q, mv[i], &r, ch);
+    rev[i] = r;
+    for (int k = 0; k < 125; k++) out[(size_t)i * 125 + k] = q[k];
+}
+
+// Features of n patches (tests).
+__global__ void lp_patch_features_kernel(const uint8_t* __restrict__ in, int n, int8_t* __restrict__ xo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t q[125];
+    uint16_t x[LP_NIN];
+    for (int k = 0; k < 125; k++) q[k] = in[(size_t)i * 125 + k];
+    lp_features(q, x);
+    for (int c = 0; c < LP_NIN; c++) xo[(size_t)i * LP_NIN + c] = x[c] ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- learned MC
+// One time unit of the learned-proposal engine on a 16^3 tile (cubic
+// sub-lattice, 64 slots, 256 threads): n_iter iterations of 4 substeps; per
+// substep every slot proposes m ~ q(. | S), evaluates q(. | S') after the
+// move and accepts with min(1, q(rev | S') / q(m | S)).
+// Randomness: Philox key (seed, stream); site offsets counter (tau, tile,
+// 0xFFFFFFFF, iter/4) word iter%4 byte s (as the plain engine); per slot
+// counter (tau, tile, slot | 0x40000000, iter) words 2s, 2s+1 = (u_move, u_accept)
+// for s < 2 from the first call and s >= 2 from counter word 3 + 1.
+struct LpMcShared {
+    LpShared F;
+    uint8_t tile[4096];
+    uint8_t qS[64][128];
+    float pS[64][LP_NMOVE + 1];
+    int mv[64], rv[64], ok[64];
+    int ch[64][3];
+};
+
+__global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ glat, RingLaunch p,
+                                                          const float* __restrict__ prm, unsigned long long* __restrict__ stats) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    LpMcShared& M = *reinterpret_cast<LpMcShared*>(lds_raw);
+    LpShared& S = M.F;
+    constexpr int TT = 16;
+    const int tid = threadIdx.x;
+    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TT, gv0 = p.ov + kv * TT;
+    lp_load_net(S, prm);
+    for (int i = tid; i < 4096; i += 256) {
+        int gt = gt0 + (i & 15), gu = gu0 + ((i >> 4) & 15), gv = gv0 + (i >> 8);
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        M.tile[i] = glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+    unsigned long long nacc = 0, nprop = 0;
+    __syncthreads();
+    for (int it = 0; it < p.n_iter; it++) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it >> 2)), p.seed, p.stream);
+        const uint32_t offw = word_of(ow, it & 3);
+        uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0;
+        if (tid < 64) {
+            r0 = philox4x32_10(make_uint4(p.tau, tileId, (uint32_t)tid | 0x40000000u, 2u * (uint32_t)it), p.seed, p.stream);
+            r1 = philox4x32_10(make_uint4(p.tau, tileId, (uint32_t)tid | 0x40000000u, 2u * (uint32_t)it + 1u), p.seed,
+                               p.stream);
+        }
+        for (int s = 0; s < 4; s++) {
+            const uint32_t c = (offw >> (8 * s)) & 0x3fu;
+            // patches of all 64 slots, by all threads: slot e / 125, entry e % 125
+            for (int e = tid; e < 64 * 125; e += 256) {
+                const int sl = e / 125, i = e - 125 * sl;
+                const int a = 4 * (sl & 3) + (int)(c & 3u) + i % 5 - 2;
+                const int b = 4 * ((sl >> 2) & 3) + (int)((c >> 2) & 3u) + (i / 5) % 5 - 2;
+                const int d = 4 * (sl >> 4) + (int)((c >> 4) & 3u) + i / 25 - 2;
+                const uint8_t v = ((unsigned)a < 16u && (unsigned)b < 16u && (unsigned)d < 16u)
+                                      ? M.tile[a + 16 * (b + 16 * d)] : WALL;
+                M.qS[sl][i] = v;
+                S.X[sl][i] = v ? BF_ONE : 0;
+            }
+            __syncthreads();
+            if (tid < 64) lp_features_tail(M.qS[tid], S.X[tid]);
+            __syncthreads();
+            lp_forward(S);
+            // propose m ~ q(. | S), apply it to the patch, features of S'
+            if (tid < 64) {
+                lp_softmax(S.Z[tid], M.pS[tid]);
+                const uint32_t um = (s < 2) ? word_of(r0, 2 * s) : word_of(r1, 2 * (s - 2));
+                float u = (float)(um >> 8) * (1.f / 16777216.f);
+                int m = LP_NMOVE - 1;
+                for (int k = 0; k < LP_NMOVE; k++) {
+                    if (u < M.pS[tid][k]) { m = k; break; }
+                    u -= M.pS[tid][k];
+                }
+                int rv = -1, ch[3] = {PC, PC, PC};
+                const int ok = lp_move(M.qS[tid], m, &rv, ch);
+                M.ok[tid] = ok;
+                M.mv[tid] = m;
+                M.rv[tid] = rv;
+                for (int j = 0; j < 3; j++) {
+                    M.ch[tid][j] = ch[j];
+                    if (ok) S.X[tid][ch[j]] = M.qS[tid][ch[j]] ? BF_ONE : 0;
+                }
+                lp_features_tail(M.qS[tid], S.X[tid]);
+            }
+            __syncthreads();
+            lp_forward(S);
+            // accept with min(1, q(rev | S') / q(m | S)); write the changed sites
+            if (tid < 64) {
+                nprop++;
+                if (M.ok[tid]) {
+                    float pT[LP_NMOVE];
+                    lp_softmax(S.Z[tid], pT);
+                    const uint32_t ua = (s < 2) ? word_of(r0, 2 * s + 1) : word_of(r1, 2 * (s - 2) + 1);
+                    const float u = (float)(ua >> 8) * (1.f / 16777216.f);
+                    if (u * M.pS[tid][M.mv[tid]] < pT[M.rv[tid]]) {
+                        nacc++;
+                        const int st = 4 * (tid & 3) + (int)(c & 3u), su = 4 * ((tid >> 2) & 3) + (int)((c >> 2) & 3u),
+                                  sv = 4 * (tid >> 4) + (int)((c >> 4) & 3u);
+                        for (int j = 0; j < 3; j++) {
+                            const int i = M.ch[tid][j];
+                            M.tile[(st + i % 5 - 2) + 16 * ((su + (i / 5) % 5 - 2) + 16 * (sv + i / 25 - 2))] = M.qS[tid][i];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < 4096; i += 256) {
+        int gt = gt0 + (i & 15), gu = gu0 + ((i >> 4) & 15), gv = gv0 + (i >> 8);
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] = M.tile[i];
+    }
+    if (stats && tid < 64) {
+        atomicAdd(&stats[0], nprop);
+        atomicAdd(&stats[1], nacc);
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+template <class K>
+static int set_lds(K kernel, size_t lds) {
+    return (int)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+int lp_launch_forward(const float* prm, const int8_t* x, int n, float* prob, hipStream_t st) {
+    const size_t lds = sizeof(LpShared);
+    int rc = set_lds(lp_forward_kernel, lds);
+    if (rc) return rc;
+    const int blocks = n / 64 + 1 < 1024 ? n / 64 + 1 : 1024;
+    hipLaunchKernelGGL(lp_forward_kernel, dim3(blocks), dim3(256), lds, st, prm, x, n, prob);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_grad(const float* prm, const int8_t* x, const int32_t* y, int n, float scale, float* grad, float* loss,
+                   hipStream_t st) {
+    const size_t lds = sizeof(LpTrainShared);
+    int rc = set_lds(lp_grad_kernel, lds);
+    if (rc) return rc;
+    const int blocks = n / 64 + 1 < 512 ? n / 64 + 1 : 512;
+    hipLaunchKernelGGL(lp_grad_kernel, dim3(blocks), dim3(256), lds, st, prm, x, y, n, scale, grad, loss);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_adam(float* prm, float* grad, float* m1, float* m2, int n, float lr, float b1, float b2, float eps,
+                   int step, hipStream_t st) {
+    const float c1 = 1.f / (1.f - powf(b1, (float)step)), c2 = 1.f / (1.f - powf(b2, (float)step));
+    hipLaunchKernelGGL(lp_adam_kernel, dim3((n + 255) / 256), dim3(256), 0, st, prm, grad, m1, m2, n, lr, b1, b2, eps,
+                       c1, c2);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_fill(float* p, int n, float v, hipStream_t st) {
+    hipLaunchKernelGGL(lp_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_harvest(const uint8_t* lat, int LT, int LU, int LV, int n, uint32_t seed, uint32_t stream, int8_t* x,
+                      int32_t* y, hipStream_t st) {
+    const int blocks = (n + 63) / 64 < 4096 ? (n + 63) / 64 : 4096;
+    hipLaunchKernelGGL(lp_harvest_kernel, dim3(blocks), dim3(64), 0, st, lat, LT, LU, LV, n, seed, stream, x, y);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_patch_move(const uint8_t* in, const int32_t* mv, int n, uint8_t* out, int32_t* ok, int32_t* rev,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(lp_patch_move_kernel, dim3((n + 63) / 64), dim3(64), 0, st, in, mv, n, out, ok, rev);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_patch_features(const uint8_t* in, int n, int8_t* x, hipStream_t st) {
+    hipLaunchKernelGGL(lp_patch_features_kernel, dim3((n + 63) / 64), dim3(64), 0, st, in, n, x);
+    return (int)hipGetLastError();
+}
+
+int lp_launch_mc(uint8_t* glat, const RingLaunch& p, int nblocks, const float* prm, unsigned long long* stats,
+                 hipStream_t st) {
+    const size_t lds = sizeof(LpMcShared);
+    int rc = set_lds(lp_ring_time_unit, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(lp_ring_time_unit, dim3(nblocks), dim3(256), lds, st, glat, p, prm, stats);
+    return (int)hipGetLastError();
+}
+
+}  // namespace elp

@@ -16,6 +16,9 @@ from .core import (  # noqa: F401
     ElpError,
     Schedule,
     autotune,
+    Net,
+    patch_moves,
+    patch_features,
     Batch,
     Engine,
     RingHost,

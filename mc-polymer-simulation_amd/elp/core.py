@@ -99,6 +99,18 @@ def load_elp():
         lib.elp_set_threads.argtypes = [P, I]
         lib.elp_batch_set_rules.argtypes = [P, I]
         lib.elp_set_rules.argtypes = [P, I]
+        lib.elp_net_create.argtypes = [C.POINTER(P), I, U32]
+        lib.elp_net_destroy.argtypes = [P]
+        lib.elp_net_params.argtypes = [P, P, U64]
+        lib.elp_net_set_params.argtypes = [P, P, U64]
+        lib.elp_net_forward.argtypes = [P, P, I, P]
+        lib.elp_net_gradient.argtypes = [P, P, P, I, P, C.POINTER(C.c_float)]
+        lib.elp_net_harvest.argtypes = [P, P, I, U32]
+        lib.elp_net_samples.argtypes = [P, C.POINTER(I), P, P]
+        lib.elp_net_train.argtypes = [P, I, I, C.c_float, C.POINTER(C.c_float)]
+        lib.elp_run_learned.argtypes = [P, P, I, P, U64, U32, U32, C.POINTER(U64), C.POINTER(U64)]
+        lib.elp_net_patch_moves.argtypes = [I, P, P, I, P, P, P]
+        lib.elp_net_patch_features.argtypes = [I, P, I, P]
         lib.elp_autotune.argtypes = [I, I, I, P, P, I, C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_schedule_format.argtypes = [C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_schedule_parse.argtypes = [C.c_char_p, C.POINTER(Schedule)]
@@ -264,6 +276,98 @@ class Batch:
 
     def set_rules(self, rules):
         _check(self.lib.elp_batch_set_rules(self.h, int(rules)), self.lib)
+
+
+NET_NPARAM = 14416
+NET_NIN = 144
+NET_NMOVE = 11
+
+
+class Net:
+    """Learned move proposal (elp_net_* in include/elp_gpu.h)."""
+
+    def __init__(self, seed=0, device=0):
+        self.lib = load_elp()
+        h = C.c_void_p()
+        _check(self.lib.elp_net_create(C.byref(h), device, int(seed) & 0xFFFFFFFF), self.lib)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.elp_net_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def params(self):
+        p = np.empty(NET_NPARAM, np.float32)
+        _check(self.lib.elp_net_params(self.h, p.ctypes.data, p.size), self.lib)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(self.lib.elp_net_set_params(self.h, p.ctypes.data, p.size), self.lib)
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.int8).reshape(-1, NET_NIN)
+        prob = np.empty((len(x), NET_NMOVE), np.float32)
+        _check(self.lib.elp_net_forward(self.h, x.ctypes.data, len(x), prob.ctypes.data), self.lib)
+        return prob
+
+    def gradient(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.int8).reshape(-1, NET_NIN)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        g = np.empty(NET_NPARAM, np.float32)
+        loss = C.c_float()
+        _check(self.lib.elp_net_gradient(self.h, x.ctypes.data, y.ctypes.data, len(x), g.ctypes.data,
+                                         C.byref(loss)), self.lib)
+        return g, loss.value
+
+    def harvest(self, engine, count, seed=0):
+        _check(self.lib.elp_net_harvest(self.h, engine.h, int(count), int(seed) & 0xFFFFFFFF), self.lib)
+
+    def samples(self):
+        n = C.c_int()
+        _check(self.lib.elp_net_samples(self.h, C.byref(n), None, None), self.lib)
+        x = np.empty((n.value, NET_NIN), np.int8)
+        y = np.empty(n.value, np.int32)
+        _check(self.lib.elp_net_samples(self.h, C.byref(n), x.ctypes.data, y.ctypes.data), self.lib)
+        return x, y
+
+    def train(self, epochs=1, batch=4096, lr=1e-3):
+        loss = C.c_float()
+        _check(self.lib.elp_net_train(self.h, int(epochs), int(batch), float(lr), C.byref(loss)), self.lib)
+        return loss.value
+
+    def run(self, engine, origins, tau0, seed, stream=0):
+        """Learned-proposal time units on a ring Engine (16^3 tiles, cubic);
+        returns (proposals, accepted)."""
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+        npr, nac = C.c_uint64(), C.c_uint64()
+        _check(self.lib.elp_run_learned(engine.h, self.h, len(o), o.ctypes.data, int(tau0), int(seed) & 0xFFFFFFFF,
+                                        int(stream) & 0xFFFFFFFF, C.byref(npr), C.byref(nac)), self.lib)
+        return npr.value, nac.value
+
+
+def patch_moves(patches, moves, device=0):
+    """Move m at the centre of each 5^3 patch: (patches after, ok, reverse move)."""
+    lib = load_elp()
+    p = np.ascontiguousarray(patches, dtype=np.uint8).reshape(-1, 125)
+    m = np.ascontiguousarray(moves, dtype=np.int32)
+    out = np.empty_like(p)
+    ok = np.empty(len(p), np.int32)
+    rev = np.empty(len(p), np.int32)
+    _check(lib.elp_net_patch_moves(device, p.ctypes.data, m.ctypes.data, len(p), out.ctypes.data, ok.ctypes.data,
+                                   rev.ctypes.data), lib)
+    return out, ok, rev
+
+
+def patch_features(patches, device=0):
+    lib = load_elp()
+    p = np.ascontiguousarray(patches, dtype=np.uint8).reshape(-1, 125)
+    x = np.empty((len(p), NET_NIN), np.int8)
+    _check(lib.elp_net_patch_features(device, p.ctypes.data, len(p), x.ctypes.data), lib)
+    return x
 
 
 class _Host:
