@@ -22,6 +22,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "mc-polymer-simulation_amd")
 sys.path.insert(0, PKG)
@@ -157,6 +159,101 @@ def bench_batch(args, tile):
     return 0
 
 
+def _rg2_now(host, eng, series_cls):
+    host.set_lattice(eng.download())
+    s = series_cls(True, host.L, host.n_pol, host.N, 1)
+    s.add_host(host)
+    return float(np.mean(s.rg2(0)))
+
+
+def bench_learned(args):
+    """Equilibration A/B: ring melt L=128, N=4000, density 1.2 (629 rings) from
+    the compact start, plain engine (balanced rates, 16^3 dense, 13
+    iterations) vs the learned proposal (16^3 cubic, 51 iterations, network
+    trained on 1e6 moves harvested from an L=96 N=1000 melt run by the plain
+    engine), each for the same wall-clock budget; Rg^2 sampled along the way.
+    value = wall-time ratio plain / learned to reach the Rg^2 both reach."""
+    import elp
+    from elp.stats import Series
+    budget = args.budget
+    tile = (16, 16, 16)
+    # training data: L=96, N=1000 melt run by the plain engine (balanced rates)
+    h96 = elp.RingHost(N=1000, L=96, density=1.2, seed=0)
+    e96 = elp.Engine("ring", h96.L, tile, elp.SUBLAT_DENSE, 13)
+    e96.set_rules(elp.RULES_BALANCED)
+    e96.upload(h96.lattice())
+    h96.load_shift(tile)
+    t0 = time.perf_counter()
+    e96.run(h96.origins(tile, 20000), tau0=0, seed=h96.seed)
+    net = elp.Net(seed=1)
+    for k in range(4):
+        e96.run(h96.origins(tile, 2000), tau0=20000 + 2000 * k, seed=h96.seed)
+        net.harvest(e96, 250000, seed=100 + k)
+    loss = net.train(epochs=4, batch=4096, lr=3e-3)
+    t_train = time.perf_counter() - t0
+    e96.close()
+    curves = {}
+    for kind in ("plain", "learned"):
+        h = elp.RingHost(N=4000, L=128, density=1.2, seed=0)
+        sub, n_iter = (elp.SUBLAT_DENSE, 13) if kind == "plain" else (elp.SUBLAT_CUBIC, 51)
+        eng = elp.Engine("ring", h.L, tile, sub, n_iter)
+        eng.set_rules(elp.RULES_BALANCED)
+        eng.upload(h.lattice())
+        h.load_shift(tile)
+        pts = [(0.0, 0, _rg2_now(h, eng, Series))]
+        tau, gpu_s, chunk = 0, 0.0, (2000 if kind == "plain" else 20)
+        while gpu_s < budget:
+            org = h.origins(tile, chunk)
+            t1 = time.perf_counter()
+            if kind == "plain":
+                eng.run(org, tau0=tau, seed=h.seed)
+            else:
+                net.run(eng, org, tau0=tau, seed=h.seed)
+            gpu_s += time.perf_counter() - t1
+            tau += chunk
+            pts.append((gpu_s, tau, _rg2_now(h, eng, Series)))
+            chunk = min(chunk * 2, 64000 if kind == "plain" else 640)
+        curves[kind] = pts
+        eng.close()
+    target = min(curves["plain"][-1][2], curves["learned"][-1][2])
+
+    def t_reach(pts):
+        for (ta, ua, ra), (tb, ub, rb) in zip(pts, pts[1:]):
+            if rb >= target:
+                f = (target - ra) / (rb - ra) if rb > ra else 1.0
+                return ta + f * (tb - ta), ua + f * (ub - ua)
+        return pts[-1][0], pts[-1][1]
+
+    (tp, up), (tl, ul) = t_reach(curves["plain"]), t_reach(curves["learned"])
+    rec = {
+        "metric": "MC move attempts/s, %HBM roofline; equil. wall-time learned vs plain",
+        "value": tp / tl,
+        "unit": "wall-time ratio plain/learned to reach the common Rg^2 (>1: learned faster)",
+        "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8 lattice, bf16/fp32 network",
+        "data": "synthetic",
+        "config": {
+            "workload": "ring melt L=128 N=4000 density 1.2 (629 rings) from the compact start; equilibration A/B "
+                        f"with {budget:.0f} s of GPU wall time each",
+            "plain": "balanced rates, 16^3 tiles, dense sub-lattice, 13 iterations",
+            "learned": "balanced-rate MH chain with the learned proposal, 16^3 tiles, cubic sub-lattice, 51 iterations",
+            "training": f"1e6 moves harvested from an L=96 N=1000 melt (plain engine, 28000 time units), 4 epochs "
+                        f"Adam, final cross-entropy {loss:.3f}, {t_train:.1f} s including the L=96 run",
+            "target_rg2": target,
+            "plain_reach": {"seconds": tp, "time_units": up},
+            "learned_reach": {"seconds": tl, "time_units": ul},
+            "time_units_ratio_plain_over_learned": up / ul if ul else None,
+            "curves": {k: [[round(a, 3), b, round(c, 4)] for a, b, c in v] for k, v in curves.items()},
+            "parallelism": "single GPU",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec))
+    net.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,9 +270,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-only", action="store_true",
                     help="only run the plain copy kernel (byte-counter calibration)")
-    ap.add_argument("--workload", default="melt", choices=["melt", "batch"],
+    ap.add_argument("--workload", default="melt", choices=["melt", "batch", "learned"],
                     help="melt: one box (default); batch: 64 ring replicas, L in {32,48,64,96}, "
-                         "N in {50,200,1000,2000}, seeds 0..63, in one launch per time unit")
+                         "N in {50,200,1000,2000}, seeds 0..63, in one launch per time unit; learned: "
+                         "equilibration A/B of the learned proposal against the plain engine")
+    ap.add_argument("--budget", type=float, default=30.0, help="learned: GPU seconds per arm")
     ap.add_argument("--units-per-graph", type=int, default=16,
                     help="batch: time units captured per HIP graph (0 = direct launches)")
     ap.add_argument("--threads", type=int, default=0, help="threads per workgroup (0 = schedule default)")
@@ -200,6 +299,8 @@ def main():
     tile = tuple(int(x) for x in args.tile.split("x")) if "x" in args.tile else (int(args.tile),) * 3
     if args.workload == "batch":
         return bench_batch(args, tile)
+    if args.workload == "learned":
+        return bench_learned(args)
     dense = args.sublattice == "dense"
     ring = args.chain == "ring"
     substeps = 4 if ring else 2

@@ -250,3 +250,83 @@ def test_plain_engine_six_ring_measure():
         else:
             assert p < 1e-6
         bat.close()
+
+
+def test_learned_statics_match_plain_balanced_engine():
+    """Ring melts L=32, N=100, 8 seeds: equilibrated by the plain engine
+    (balanced rates, 20000 time units), then 1500 more time units of either
+    the plain engine or the learned proposal (trained on moves harvested from
+    the plain engine), frames every 100.  Rg^2, stored-length ratio and P(s)
+    agree within 3 sigma."""
+    from elp.stats import Series
+    tile = (16, 16, 16)
+    net = elp.Net(seed=5)
+    res = {"plain": [], "learned": []}
+    acc = prop = 0
+    for seed in range(8):
+        _say(f"[learned statics] seed {seed}")
+        h = elp.RingHost(N=100, L=32, density=1.2, seed=seed)
+        eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_CUBIC, 51)
+        eng.set_rules(elp.RULES_BALANCED)
+        eng.upload(h.lattice())
+        h.load_shift(tile)
+        eng.run(h.origins(tile, 20000), tau0=0, seed=h.seed)
+        lat_eq = eng.download()
+        if seed == 0:
+            net.harvest(eng, 300000, seed=17)
+            net.train(epochs=4, batch=4096, lr=3e-3)
+        for kind in ("plain", "learned"):
+            eng.upload(lat_eq)
+            h.set_lattice(lat_eq)
+            ser = Series(True, h.L, h.n_pol, h.N, 16)
+            tau = 20000
+            for k in range(15):
+                org = h.origins(tile, 100)
+                if kind == "plain":
+                    eng.run(org, tau0=tau, seed=h.seed)
+                else:
+                    p_, a_ = net.run(eng, org, tau0=tau, seed=h.seed ^ 0x5A5A)
+                    prop, acc = prop + p_, acc + a_
+                tau += 100
+                h.set_lattice(eng.download())
+                ser.add_host(h)
+            st = ser.statics(0)
+            res[kind].append((st["rg2"], st["slrat"], *st["pc"][[2, 5, 10, 25, 50]]))
+        eng.close()
+    a, b = np.array(res["plain"]), np.array(res["learned"])
+    z = (a.mean(0) - b.mean(0)) / np.sqrt(a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b))
+    _say(f"  plain   {np.round(a.mean(0), 4)}\n  learned {np.round(b.mean(0), 4)}\n  z {np.round(z, 2)}; "
+         f"learned acceptance {acc / prop:.4f}")
+    assert np.all(np.abs(z) < 3), z
+    net.close()
+
+
+def test_trained_network_raises_accepted_moves_per_attempt():
+    """What the trained network gains over the same chain without it: on an
+    equilibrated melt (L=32, N=100), the fraction of proposals accepted with
+    the trained network against a uniform proposal over the 11 moves (all
+    weights zero)."""
+    tile = (16, 16, 16)
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=21)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_CUBIC, 51)
+    eng.set_rules(elp.RULES_BALANCED)
+    eng.upload(h.lattice())
+    h.load_shift(tile)
+    eng.run(h.origins(tile, 20000), tau0=0, seed=h.seed)
+    lat_eq = eng.download()
+    trained = elp.Net(seed=6)
+    trained.harvest(eng, 300000, seed=23)
+    trained.train(epochs=4, batch=4096, lr=3e-3)
+    uniform = elp.Net(seed=7)
+    uniform.set_params(np.zeros(NPARAM, np.float32))
+    rates = {}
+    for name, net in (("uniform", uniform), ("trained", trained)):
+        eng.upload(lat_eq)
+        p_, a_ = net.run(eng, h.origins(tile, 200), tau0=30000, seed=99)
+        rates[name] = a_ / p_
+    _say(f"  accepted per proposal: uniform {rates['uniform']:.4f}, trained {rates['trained']:.4f} "
+         f"({rates['trained'] / rates['uniform']:.2f}x)")
+    assert rates["trained"] > 2.0 * rates["uniform"]
+    trained.close()
+    uniform.close()
+    eng.close()
