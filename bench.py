@@ -148,7 +148,7 @@ def bench_batch(args, tile):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("ring_batch_unit", "ring", "batch64", "mixed", tile, args.sublattice),
             "kernel": "ring_batch_unit",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": launch_s * 1e6,
