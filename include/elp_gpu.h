@@ -191,6 +191,25 @@ int elp_schedule_format(const elp_schedule* s, char* buf, uint64_t len);
 int elp_schedule_parse(const char* text, elp_schedule* s);
 
 /*
+ * Ring recovery on the device (replaces the host walk GetAllRingPolymers /
+ * FindPolymerStart / GetRingPolymer / UpdatePolymerWithLabels,
+ * gpupol2/oclpol.c:183-376, and the writer WriteLatticeFile,
+ * gpupol2/gpupol_io.c:167-203, with byte-identical output).  label2id: the
+ * host's table of 2^18 entries (18 id bits of a ring -> polymer id, -1 if
+ * unused); n_pol rings of at most max_len monomers.
+ * elp_ring_chains: per polymer id its bond string (0 = stored length) in
+ *   bonds[id*max_len ..], its length and its start site (3 ints); any output
+ *   may be NULL.
+ * elp_ring_tuv: the whole tuv file text (header with the given maxPolLength,
+ *   then one line per ring in id order); with text == NULL only *len is set.
+ * Both return ELP_E_STATE when the lattice does not hold n_pol labelled rings.
+ */
+int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds,
+                    int32_t* lens, int32_t* starts);
+int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
+                 char* text, uint64_t cap, uint64_t* len);
+
+/*
  * Learned proposal for ring equilibration (no reference counterpart).
  * A small network (144 int8 features of a site's 5^3 neighbourhood and bond
  * -> 64 -> 64 -> 11 move logits; bf16 weights on MFMA, fp32 accumulation,

@@ -1,0 +1,161 @@
+// elp_chains.hip -- ring recovery and tuv text on the GPU.
+//
+// Replaces the host walk of GetAllRingPolymers / FindPolymerStart /
+// GetRingPolymer / UpdatePolymerWithLabels (gpupol2/oclpol.c:183-376) and the
+// per-character writer WriteLatticeFile (gpupol2/gpupol_io.c:167-203) for the
+// ring engine, with the same result byte for byte:
+//   * monomers are visited in chain order: on a site holding stored length
+//     the stored-length monomer (label bit 5) comes first, then the monomer
+//     carrying the site's bond (label bit 4), then the next site;
+//   * a ring starts on the monomer after its label head 1,0,0,0,0; its id is
+//     label2id[the 18 label bits that follow, most significant first];
+//   * its bond string lists, from the start, 0 for a stored-length monomer
+//     and the bond code otherwise; the tuv line is
+//     "len= N\n" "t  u  v\t" hex string "\n", rings in id order.
+// One thread per site finds heads (L^3 threads, five-monomer look-ahead),
+// one thread per ring walks it, one thread per ring formats its line.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "elp_device.h"
+#include "elp_chains.h"
+
+namespace elp {
+
+struct Cursor {
+    uint32_t site;
+    uint32_t sl;      // 1: the stored-length monomer of the site, 0: its bond carrier
+};
+
+struct Box {
+    int LT, LU, LV;
+    __device__ uint32_t step(uint32_t x, uint32_t b) const {
+        int t = (int)(x % (uint32_t)LT), u = (int)((x / (uint32_t)LT) % (uint32_t)LU), v = (int)(x / ((uint32_t)LT * LU));
+        t += bdt(b); u += bdu(b); v += bdv(b);
+        t += (t < 0) ? LT : 0; t -= (t >= LT) ? LT : 0;
+        u += (u < 0) ? LU : 0; u -= (u >= LU) ? LU : 0;
+        v += (v < 0) ? LV : 0; v -= (v >= LV) ? LV : 0;
+        return (uint32_t)t + (uint32_t)LT * ((uint32_t)u + (uint32_t)LU * (uint32_t)v);
+    }
+};
+
+__device__ __forceinline__ uint32_t label_of(const uint8_t* lat, Cursor c) {
+    const uint32_t s = lat[c.site];
+    return c.sl ? (s >> 5) & 1u : (s >> 4) & 1u;
+}
+
+__device__ __forceinline__ Cursor next_of(const uint8_t* lat, const Box& B, Cursor c) {
+    if (c.sl) return Cursor{c.site, 0u};
+    const uint32_t n = B.step(c.site, lat[c.site] & 0xfu);
+    return Cursor{n, (lat[n] >> 6) & 1u};
+}
+
+// Starts of all rings: the monomer after each head 1,0,0,0,0.
+__global__ void ring_heads_kernel(const uint8_t* __restrict__ lat, Box B, uint32_t nsite, uint32_t* __restrict__ starts,
+                                  uint32_t cap, uint32_t* __restrict__ count) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsite) return;
+    const uint32_t s = lat[x];
+    if (!(s & 0xfu)) return;
+    for (uint32_t k = 0; k < 2; k++) {
+        Cursor c{x, k == 0 ? 1u : 0u};
+        if (c.sl && !(s & 0x40u)) continue;
+        if (!label_of(lat, c)) continue;
+        bool head = true;
+        for (int i = 0; i < 4 && head; i++) {
+            c = next_of(lat, B, c);
+            head = label_of(lat, c) == 0u;
+        }
+        if (!head) continue;
+        c = next_of(lat, B, c);
+        const uint32_t i = atomicAdd(count, 1u);
+        if (i < cap) starts[i] = (c.site << 1) | c.sl;
+    }
+}
+
+// Walk every ring from its start: id from the 18 label bits, bond string,
+// length, start site and the length of its tuv line.
+__global__ void ring_walk_kernel(const uint8_t* __restrict__ lat, Box B, const uint32_t* __restrict__ starts,
+                                 uint32_t n_start, const int32_t* __restrict__ label2id, int n_pol, int max_len,
+                                 uint8_t* __restrict__ bonds, int32_t* __restrict__ lens, int32_t* __restrict__ start3,
+                                 int32_t* __restrict__ line_len, uint32_t* __restrict__ err) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_start) return;
+    const Cursor c0{starts[r] >> 1, starts[r] & 1u};
+    Cursor c = c0;
+    uint32_t lab = 0;
+    for (int i = 0; i < 18; i++) {
+        lab = (lab << 1) | label_of(lat, c);
+        c = next_of(lat, B, c);
+    }
+    const int id = label2id[lab];
+    if (id < 0 || id >= n_pol) { atomicOr(err, 1u); return; }
+    c = c0;
+    int n = 0;
+    do {
+        if (n >= max_len) { atomicOr(err, 2u); return; }
+        bonds[(size_t)id * max_len + n] = c.sl ? 0 : (uint8_t)(lat[c.site] & 0xfu);
+        n++;
+        c = next_of(lat, B, c);
+    } while (!(c.site == c0.site && c.sl == c0.sl));
+    lens[id] = n;
+    const int t = (int)(c0.site % (uint32_t)B.LT), u = (int)((c0.site / (uint32_t)B.LT) % (uint32_t)B.LU),
+              v = (int)(c0.site / ((uint32_t)B.LT * B.LU));
+    start3[3 * id] = t; start3[3 * id + 1] = u; start3[3 * id + 2] = v;
+    auto digits = [](int a) { int d = 1; while (a >= 10) { a /= 10; d++; } return d; };
+    // "len= " N "\n" t "  " u "  " v "\t" hex*N "\n"
+    line_len[id] = 5 + digits(n) + 1 + digits(t) + 2 + digits(u) + 2 + digits(v) + 1 + n + 1;
+}
+
+__device__ char* put_int(char* p, int a) {
+    char tmp[12];
+    int k = 0;
+    do { tmp[k++] = (char)('0' + a % 10); a /= 10; } while (a);
+    while (k) *p++ = tmp[--k];
+    return p;
+}
+
+__global__ void ring_text_kernel(const uint8_t* __restrict__ bonds, const int32_t* __restrict__ lens,
+                                 const int32_t* __restrict__ start3, const uint64_t* __restrict__ offset, int n_pol,
+                                 int max_len, char* __restrict__ text) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_pol) return;
+    char* p = text + offset[id];
+    const int n = lens[id];
+    *p++ = 'l'; *p++ = 'e'; *p++ = 'n'; *p++ = '='; *p++ = ' ';
+    p = put_int(p, n);
+    *p++ = '\n';
+    p = put_int(p, start3[3 * id]);
+    *p++ = ' '; *p++ = ' ';
+    p = put_int(p, start3[3 * id + 1]);
+    *p++ = ' '; *p++ = ' ';
+    p = put_int(p, start3[3 * id + 2]);
+    *p++ = '\t';
+    const uint8_t* b = bonds + (size_t)id * max_len;
+    for (int i = 0; i < n; i++) *p++ = "0123456789abcdef"[b[i] & 0xf];
+    *p = '\n';
+}
+
+int chains_launch_heads(const uint8_t* lat, int LT, int LU, int LV, uint32_t* starts, uint32_t cap, uint32_t* count,
+                        hipStream_t st) {
+    const uint32_t n = (uint32_t)LT * LU * LV;
+    hipLaunchKernelGGL(ring_heads_kernel, dim3((n + 255) / 256), dim3(256), 0, st, lat, Box{LT, LU, LV}, n, starts, cap,
+                       count);
+    return (int)hipGetLastError();
+}
+
+int chains_launch_walk(const uint8_t* lat, int LT, int LU, int LV, const uint32_t* starts, uint32_t n_start,
+                       const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens, int32_t* start3,
+                       int32_t* line_len, uint32_t* err, hipStream_t st) {
+    hipLaunchKernelGGL(ring_walk_kernel, dim3((n_start + 63) / 64), dim3(64), 0, st, lat, Box{LT, LU, LV}, starts,
+                       n_start, label2id, n_pol, max_len, bonds, lens, start3, line_len, err);
+    return (int)hipGetLastError();
+}
+
+int chains_launch_text(const uint8_t* bonds, const int32_t* lens, const int32_t* start3, const uint64_t* offset,
+                       int n_pol, int max_len, char* text, hipStream_t st) {
+    hipLaunchKernelGGL(ring_text_kernel, dim3((n_pol + 63) / 64), dim3(64), 0, st, bonds, lens, start3, offset, n_pol,
+                       max_len, text);
+    return (int)hipGetLastError();
+}
+
+}  // namespace elp

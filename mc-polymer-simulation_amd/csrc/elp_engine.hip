@@ -10,6 +10,7 @@
 #include <vector>
 #include <math.h>
 #include "../../include/elp_gpu.h"
+#include "elp_chains.h"
 #include "elp_kernels.h"
 #include "elp_learned.h"
 
@@ -845,6 +846,98 @@ int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origin
     HIPCHK(hipStreamSynchronize(e->stream));
     if (proposals) *proposals = st[0];
     if (accepted) *accepted = st[1];
+    return ELP_OK;
+}
+
+// Ring recovery on the device: per ring (in id order) its bond string, length
+// and start site; optionally the tuv text.  Shared by elp_ring_chains and
+// elp_ring_tuv.
+struct ChainBufs {
+    DevBuf l2i, starts, count, bonds, lens, start3, line_len, err, offset, text;
+};
+
+static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
+    if (!e || !label2id || n_pol <= 0 || max_len <= 0) return fail(ELP_E_ARG, "bad chain-recovery arguments");
+    if (e->kind != 0) return fail(ELP_E_ARG, "chain recovery needs a ring engine");
+    if (!e->uploaded) return fail(ELP_E_STATE, "chain recovery before upload");
+    HIPCHK(hipSetDevice(e->device));
+    const size_t nl = (size_t)1 << 18;
+    HIPCHK(hipMalloc(&B.l2i.p, sizeof(int32_t) * nl));
+    HIPCHK(hipMalloc(&B.starts.p, sizeof(uint32_t) * (size_t)n_pol));
+    HIPCHK(hipMalloc(&B.count.p, sizeof(uint32_t) * 2));
+    HIPCHK(hipMalloc(&B.bonds.p, (size_t)n_pol * max_len));
+    HIPCHK(hipMalloc(&B.lens.p, sizeof(int32_t) * n_pol));
+    HIPCHK(hipMalloc(&B.start3.p, sizeof(int32_t) * 3 * n_pol));
+    HIPCHK(hipMalloc(&B.line_len.p, sizeof(int32_t) * n_pol));
+    HIPCHK(hipMemcpyAsync(B.l2i.p, label2id, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
+    uint32_t zero[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(B.count.p, zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
+    std::vector<int32_t> neg(n_pol, 0);
+    HIPCHK(hipMemcpyAsync(B.lens.p, neg.data(), sizeof(int32_t) * n_pol, hipMemcpyHostToDevice, e->stream));
+    uint32_t* cnt = (uint32_t*)B.count.p;
+    int rc = elp::chains_launch_heads((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
+                                      (uint32_t)n_pol, cnt, e->stream);
+    if (rc) return fail(ELP_E_HIP, "head kernel: %s", hipGetErrorString((hipError_t)rc));
+    uint32_t nfound = 0;
+    HIPCHK(hipMemcpyAsync(&nfound, cnt, sizeof nfound, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (nfound != (uint32_t)n_pol) return fail(ELP_E_STATE, "found %u ring heads, expected %d", nfound, n_pol);
+    rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p, nfound,
+                                 (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p, (int32_t*)B.lens.p,
+                                 (int32_t*)B.start3.p, (int32_t*)B.line_len.p, cnt + 1, e->stream);
+    if (rc) return fail(ELP_E_HIP, "walk kernel: %s", hipGetErrorString((hipError_t)rc));
+    uint32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, cnt + 1, sizeof err, hipMemcpyDeviceToHost, e->stream));
+    std::vector<int32_t> lens(n_pol);
+    HIPCHK(hipMemcpyAsync(lens.data(), B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (err & 1u) return fail(ELP_E_STATE, "a ring label is not a known polymer id");
+    if (err & 2u) return fail(ELP_E_STATE, "a ring is longer than %d", max_len);
+    for (int i = 0; i < n_pol; i++)
+        if (lens[i] <= 0) return fail(ELP_E_STATE, "polymer id %d found %s", i, lens[i] ? "twice" : "nowhere");
+    return ELP_OK;
+}
+
+int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens,
+                    int32_t* starts) {
+    ChainBufs B;
+    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    if (rc) return rc;
+    if (bonds) HIPCHK(hipMemcpyAsync(bonds, B.bonds.p, (size_t)n_pol * max_len, hipMemcpyDeviceToHost, e->stream));
+    if (lens) HIPCHK(hipMemcpyAsync(lens, B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    if (starts) HIPCHK(hipMemcpyAsync(starts, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return ELP_OK;
+}
+
+int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
+                 char* text, uint64_t cap, uint64_t* len) {
+    if (!len) return fail(ELP_E_ARG, "len is NULL");
+    ChainBufs B;
+    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    if (rc) return rc;
+    std::vector<int32_t> ll(n_pol);
+    HIPCHK(hipMemcpyAsync(ll.data(), B.line_len.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    char head[256];
+    const int hn = snprintf(head, sizeof head, "LT= %i\nLU= %i\nLV= %i\nnp= %i\nmaxPolLength= %i\n", e->LT, e->LU,
+                            e->LV, n_pol, header_max_pol_length);
+    std::vector<uint64_t> off(n_pol);
+    uint64_t total = 0;
+    for (int i = 0; i < n_pol; i++) { off[i] = total; total += (uint64_t)ll[i]; }
+    *len = (uint64_t)hn + total;
+    if (!text) return ELP_OK;                          // size query
+    if (cap < *len) return fail(ELP_E_ARG, "text buffer of %llu bytes, need %llu", (unsigned long long)cap,
+                                (unsigned long long)*len);
+    HIPCHK(hipMalloc(&B.offset.p, sizeof(uint64_t) * n_pol));
+    HIPCHK(hipMalloc(&B.text.p, total ? total : 1));
+    HIPCHK(hipMemcpyAsync(B.offset.p, off.data(), sizeof(uint64_t) * n_pol, hipMemcpyHostToDevice, e->stream));
+    rc = elp::chains_launch_text((const uint8_t*)B.bonds.p, (const int32_t*)B.lens.p, (const int32_t*)B.start3.p,
+                                 (const uint64_t*)B.offset.p, n_pol, max_len, (char*)B.text.p, e->stream);
+    if (rc) return fail(ELP_E_HIP, "text kernel: %s", hipGetErrorString((hipError_t)rc));
+    memcpy(text, head, (size_t)hn);
+    HIPCHK(hipMemcpyAsync(text + hn, B.text.p, total, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return ELP_OK;
 }
 

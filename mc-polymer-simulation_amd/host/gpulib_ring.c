@@ -86,6 +86,26 @@ int GPULatticeToCPU(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, 
     return 0;
 }
 
+int ElpGpuTuvEnabled(void) {
+    const char* env = getenv("ELP_GPU_TUV");
+    return !(env && env[0] == '0');
+}
+
+int WriteLatticeFileDevice(SimProperties* sp, SimState* ss, GPUDeviceState* dev, const char* file) {
+    elp_engine* e = (elp_engine*)dev->engine;
+    uint64_t len = 0;
+    const int maxLen = sp->polLength > 0 ? sp->polLength : (int)sp->maxPolLength;
+    ELPCHK(elp_ring_tuv(e, ss->label2Id, ss->nPol, maxLen, (int)sp->maxPolLength, NULL, 0, &len));
+    char* text = (char*)malloc(len ? len : 1);
+    ELPCHK(elp_ring_tuv(e, ss->label2Id, ss->nPol, maxLen, (int)sp->maxPolLength, text, len, &len));
+    FILE* f = fopen(file, "w");
+    if (!f) { printf("Error opening file %s\n", file); free(text); return -1; }
+    fwrite(text, 1, len, f);
+    fclose(f);
+    free(text);
+    return 0;
+}
+
 /* gpupol2/gpupol_cuda.cu:78-99 */
 int GPULibLoadBuffers(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx) {
     printf("Loading GPU buffers\n");
@@ -142,6 +162,14 @@ static void run_units(SimProperties* sp, SimState* ss, GPUDeviceState* devStates
 int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime) {
     run_units(sp, ss, devStates, nTime, sp->curT);
     GPULatticeToCPU(sp, ss, devStates, ctx);
+    return 0;
+}
+
+/* GPULibRun without the copy of the lattice and the chains to the host (the
+   caller writes the tuv file from the device, WriteLatticeFileDevice). */
+int GPULibRunDevice(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime) {
+    (void)ctx;
+    run_units(sp, ss, devStates, nTime, sp->curT);
     return 0;
 }
 

@@ -48,9 +48,13 @@ int main(int argc, char** argv) {
     double totStep = (double)(sp.tMax - sp.curT);
     double kernelMs = 0;
     long units = 0;
+    /* the tuv files come from the device-side ring recovery unless ELP_GPU_TUV=0
+       (fast equilibration reshuffles stored length on the host: host path) */
+    const int deviceTuv = ElpGpuTuvEnabled() && !sp.fastEq;
     for (; sp.curT < sp.tMax; sp.curT += sp.writeInterval) {
         gettimeofday(&t0, NULL);
         if (sp.fastEq) GPULibEqRun(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
+        else if (deviceTuv) GPULibRunDevice(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
         else GPULibRun(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
         gettimeofday(&t1, NULL);
         kernelMs += devices[0].lastKernelMs;
@@ -62,7 +66,8 @@ int main(int argc, char** argv) {
         fflush(stdout);
         for (int i = 0; i < sp.nDevices; i++) {
             snprintf(fileOut, sizeof fileOut, "%s/t=%li_dev=%i.res", sp.dirOut, sp.curT + sp.writeInterval, i);
-            WriteLatticeFile(&sp, ss + i, fileOut);
+            if (deviceTuv) WriteLatticeFileDevice(&sp, ss + i, devices + i, fileOut);
+            else WriteLatticeFile(&sp, ss + i, fileOut);
         }
     }
     printf("\nSimulation complete\n");

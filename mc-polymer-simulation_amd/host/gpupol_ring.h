@@ -128,6 +128,13 @@ int GPULibRelease(SimProperties* sp, GPUDeviceState* devStates, GPULibContext* c
 int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime);
 int GPULoadLattices(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx);
 int GPULatticeToCPU(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx);
+
+/* tuv file of a device's lattice, written from the device-side ring recovery
+   (elp_ring_tuv); same bytes as GPULatticeToCPU + WriteLatticeFile.  Used by
+   the drop-in unless ELP_GPU_TUV=0. */
+int WriteLatticeFileDevice(SimProperties* sp, SimState* ss, GPUDeviceState* dev, const char* file);
+int ElpGpuTuvEnabled(void);
+int GPULibRunDevice(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime);
 int GPULibEqRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime);
 
 /* Schedule of the device engine, read from the environment:

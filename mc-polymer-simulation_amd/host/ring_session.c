@@ -126,6 +126,14 @@ void elp_ring_host_origins(elp_ring_host* h, int tt, int tu, int tv, int n, int3
 
 uint32_t elp_ring_host_seed(const elp_ring_host* h) { return h->sp.seed; }
 
+/* The table of 2^18 entries mapping a ring's 18 id label bits to its polymer
+   id (-1: unused), as elp_ring_chains / elp_ring_tuv take it. */
+int elp_ring_host_label2id(const elp_ring_host* h, int32_t* out, uint64_t n) {
+    if (n != ((uint64_t)1 << h->ss.labelBit) || !h->ss.label2Id) return -1;
+    for (uint64_t i = 0; i < n; i++) out[i] = h->ss.label2Id[i];
+    return 0;
+}
+
 /* The per-lane seed table drawn by SimStateInit (gpupol2/gpupol_init.c:22-33):
    (nwg * 512 * 8) uint32, lane l of work-group w at 8*(l + 512*w): 4 words of
    its own state, then 4 words shared by the work-group.  The counter-based

@@ -1,0 +1,59 @@
+"""Ring recovery and tuv text on the device (elp_ring_chains / elp_ring_tuv)
+against the host recovery and writer (GetAllRingPolymers + WriteLatticeFile
+restated from gpupol2/oclpol.c and gpupol2/gpupol_io.c): the same bytes."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import elp
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _run(N, L, seed, n_time, tile=(16, 16, 16), sub=elp.SUBLAT_DENSE, n_iter=13):
+    h = elp.RingHost(N=N, L=L, density=1.2, seed=seed)
+    eng = elp.Engine("ring", h.L, tile, sub, n_iter)
+    eng.upload(h.lattice())
+    h.load_shift(tile)
+    eng.run(h.origins(tile, n_time), tau0=0, seed=h.seed)
+    return h, eng
+
+
+@pytest.mark.parametrize("N,L,seed,n_time", [(100, 32, 3, 50), (200, 64, 4, 20), (1000, 128, 5, 30)])
+def test_device_tuv_equals_host_writer(tmp_path, N, L, seed, n_time):
+    _say(f"[chains] N={N} L={L}")
+    h, eng = _run(N, L, seed, n_time)
+    t0 = time.perf_counter()
+    text = eng.ring_tuv(h)
+    t1 = time.perf_counter()
+    h.set_lattice(eng.download())
+    path = tmp_path / "t.res"
+    h.write_tuv(str(path))
+    t2 = time.perf_counter()
+    want = path.read_bytes()
+    _say(f"  device recovery + text {1e3 * (t1 - t0):.1f} ms, host download + recovery + writer "
+         f"{1e3 * (t2 - t1):.1f} ms, {len(want)} bytes")
+    assert text == want
+    bonds, lens, starts = eng.ring_chains(h)
+    for pid in range(0, h.n_pol, max(1, h.n_pol // 50)):
+        st, b = h.polymer(pid)
+        assert lens[pid] == len(b) and tuple(starts[pid]) == st and np.array_equal(bonds[pid, :len(b)], b)
+    eng.close()
+
+
+def test_device_recovery_rejects_a_damaged_lattice():
+    h, eng = _run(100, 32, 6, 5)
+    lat = eng.download()
+    heads = np.flatnonzero((lat & 0x10) != 0)
+    lat[heads] &= 0xEF                          # every carrier label 1 -> 0: no ring head left
+    eng.upload(lat)
+    with pytest.raises(elp.ElpError, match="ring heads"):
+        eng.ring_tuv(h)
+    eng.close()

@@ -48,9 +48,11 @@ def test_ring_dropin_matches_replay_and_resumes(tmp_path):
     h.write_tuv(tmp_path / "t100.res")
     assert filecmp.cmp(d / "t=100_dev=0.res", tmp_path / "t100.res", shallow=False)
 
-    # deterministic
+    # deterministic, and the host writer (ELP_GPU_TUV=0) gives the same files
+    # as the device-side recovery the executable uses by default
     d2 = tmp_path / "ring2"
-    _run("gpupol3_cuda_ring", [100, 200, 7, d2, 1.2, 0, 100, 32, 32, 32])
+    _run("gpupol3_cuda_ring", [100, 200, 7, d2, 1.2, 0, 100, 32, 32, 32], env=dict(os.environ, ELP_GPU_TUV="0"))
+    assert filecmp.cmp(d / "t=100_dev=0.res", d2 / "t=100_dev=0.res", shallow=False)
     assert filecmp.cmp(d / "t=200_dev=0.res", d2 / "t=200_dev=0.res", shallow=False)
 
     # resume from t=200: seed += 200/100, two more intervals
