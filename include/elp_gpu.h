@@ -208,6 +208,11 @@ int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_l
                     int32_t* lens, int32_t* starts);
 int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
                  char* text, uint64_t cap, uint64_t* len);
+/* Per polymer id, from the same device-side walk: squared radius of gyration
+   in bond units (analysis/mode_compute.c:191-217) and the number of
+   stored-length bonds (ComputeSL, :145-153); either output may be NULL. */
+int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, double* rg2,
+                         int32_t* n_sl);
 
 /*
  * Learned proposal for ring equilibration (no reference counterpart).

@@ -13,7 +13,10 @@
 //     and the bond code otherwise; the tuv line is
 //     "len= N\n" "t  u  v\t" hex string "\n", rings in id order.
 // One thread per site finds heads (L^3 threads, five-monomer look-ahead),
-// one thread per ring walks it, one thread per ring formats its line.
+// one thread per ring walks it (and sums the moments of its monomer
+// positions: Rg^2 and the stored-length count, as ComputeGyration and
+// ComputeSL of analysis/mode_compute.c:145-203), one thread per ring formats
+// its line.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "elp_device.h"
@@ -77,7 +80,8 @@ __global__ void ring_heads_kernel(const uint8_t* __restrict__ lat, Box B, uint32
 __global__ void ring_walk_kernel(const uint8_t* __restrict__ lat, Box B, const uint32_t* __restrict__ starts,
                                  uint32_t n_start, const int32_t* __restrict__ label2id, int n_pol, int max_len,
                                  uint8_t* __restrict__ bonds, int32_t* __restrict__ lens, int32_t* __restrict__ start3,
-                                 int32_t* __restrict__ line_len, uint32_t* __restrict__ err) {
+                                 int32_t* __restrict__ line_len, double* __restrict__ rg2, int32_t* __restrict__ nsl,
+                                 uint32_t* __restrict__ err) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_start) return;
     const Cursor c0{starts[r] >> 1, starts[r] & 1u};
@@ -90,14 +94,24 @@ __global__ void ring_walk_kernel(const uint8_t* __restrict__ lat, Box B, const u
     const int id = label2id[lab];
     if (id < 0 || id >= n_pol) { atomicOr(err, 1u); return; }
     c = c0;
-    int n = 0;
+    int n = 0, zeros = 0;
+    // unwrapped (t,u,v) along the ring; moments of x = t+u-v, y = t-u, z = v
+    int pt = 0, pu = 0, pv = 0;
+    double sx = 0, sy = 0, sz = 0, sq = 0;
     do {
         if (n >= max_len) { atomicOr(err, 2u); return; }
-        bonds[(size_t)id * max_len + n] = c.sl ? 0 : (uint8_t)(lat[c.site] & 0xfu);
+        const uint32_t b = c.sl ? 0u : (lat[c.site] & 0xfu);
+        bonds[(size_t)id * max_len + n] = (uint8_t)b;
+        const double x = pt + pu - pv, y = pt - pu, z = pv;
+        sx += x; sy += y; sz += z; sq += x * x + y * y + z * z;
+        if (b) { pt += bdt(b); pu += bdu(b); pv += bdv(b); } else zeros++;
         n++;
         c = next_of(lat, B, c);
     } while (!(c.site == c0.site && c.sl == c0.sl));
     lens[id] = n;
+    // squared radius of gyration in bond units (|bond|^2 = 2 in x, y, z)
+    rg2[id] = 0.5 * (sq / n - (sx * sx + sy * sy + sz * sz) / ((double)n * n));
+    nsl[id] = zeros;
     const int t = (int)(c0.site % (uint32_t)B.LT), u = (int)((c0.site / (uint32_t)B.LT) % (uint32_t)B.LU),
               v = (int)(c0.site / ((uint32_t)B.LT * B.LU));
     start3[3 * id] = t; start3[3 * id + 1] = u; start3[3 * id + 2] = v;
@@ -145,9 +159,9 @@ int chains_launch_heads(const uint8_t* lat, int LT, int LU, int LV, uint32_t* st
 
 int chains_launch_walk(const uint8_t* lat, int LT, int LU, int LV, const uint32_t* starts, uint32_t n_start,
                        const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens, int32_t* start3,
-                       int32_t* line_len, uint32_t* err, hipStream_t st) {
+                       int32_t* line_len, double* rg2, int32_t* nsl, uint32_t* err, hipStream_t st) {
     hipLaunchKernelGGL(ring_walk_kernel, dim3((n_start + 63) / 64), dim3(64), 0, st, lat, Box{LT, LU, LV}, starts,
-                       n_start, label2id, n_pol, max_len, bonds, lens, start3, line_len, err);
+                       n_start, label2id, n_pol, max_len, bonds, lens, start3, line_len, rg2, nsl, err);
     return (int)hipGetLastError();
 }
 

@@ -853,7 +853,7 @@ int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origin
 // and start site; optionally the tuv text.  Shared by elp_ring_chains and
 // elp_ring_tuv.
 struct ChainBufs {
-    DevBuf l2i, starts, count, bonds, lens, start3, line_len, err, offset, text;
+    DevBuf l2i, starts, count, bonds, lens, start3, line_len, rg2, nsl, err, offset, text;
 };
 
 static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
@@ -869,6 +869,8 @@ static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max
     HIPCHK(hipMalloc(&B.lens.p, sizeof(int32_t) * n_pol));
     HIPCHK(hipMalloc(&B.start3.p, sizeof(int32_t) * 3 * n_pol));
     HIPCHK(hipMalloc(&B.line_len.p, sizeof(int32_t) * n_pol));
+    HIPCHK(hipMalloc(&B.rg2.p, sizeof(double) * n_pol));
+    HIPCHK(hipMalloc(&B.nsl.p, sizeof(int32_t) * n_pol));
     HIPCHK(hipMemcpyAsync(B.l2i.p, label2id, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
     uint32_t zero[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(B.count.p, zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
@@ -884,7 +886,8 @@ static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max
     if (nfound != (uint32_t)n_pol) return fail(ELP_E_STATE, "found %u ring heads, expected %d", nfound, n_pol);
     rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p, nfound,
                                  (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p, (int32_t*)B.lens.p,
-                                 (int32_t*)B.start3.p, (int32_t*)B.line_len.p, cnt + 1, e->stream);
+                                 (int32_t*)B.start3.p, (int32_t*)B.line_len.p, (double*)B.rg2.p, (int32_t*)B.nsl.p,
+                                 cnt + 1, e->stream);
     if (rc) return fail(ELP_E_HIP, "walk kernel: %s", hipGetErrorString((hipError_t)rc));
     uint32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, cnt + 1, sizeof err, hipMemcpyDeviceToHost, e->stream));
@@ -906,6 +909,17 @@ int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_l
     if (bonds) HIPCHK(hipMemcpyAsync(bonds, B.bonds.p, (size_t)n_pol * max_len, hipMemcpyDeviceToHost, e->stream));
     if (lens) HIPCHK(hipMemcpyAsync(lens, B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
     if (starts) HIPCHK(hipMemcpyAsync(starts, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return ELP_OK;
+}
+
+int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, double* rg2,
+                         int32_t* n_sl) {
+    ChainBufs B;
+    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    if (rc) return rc;
+    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return ELP_OK;
 }

@@ -101,6 +101,7 @@ def load_elp():
         lib.elp_set_rules.argtypes = [P, I]
         lib.elp_ring_chains.argtypes = [P, P, I, I, P, P, P]
         lib.elp_ring_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
+        lib.elp_ring_chain_stats.argtypes = [P, P, I, I, P, P]
         lib.elp_net_create.argtypes = [C.POINTER(P), I, U32]
         lib.elp_net_destroy.argtypes = [P]
         lib.elp_net_params.argtypes = [P, P, U64]
@@ -222,6 +223,15 @@ class Engine:
         _check(self.lib.elp_ring_chains(self.h, l2i.ctypes.data, host.n_pol, host.N, bonds.ctypes.data,
                                         lens.ctypes.data, starts.ctypes.data), self.lib)
         return bonds, lens, starts
+
+    def ring_stats(self, host):
+        """Per polymer id, from the device-side walk: (Rg^2 in bond units, stored-length bond count)."""
+        l2i = host.label2id()
+        rg2 = np.zeros(host.n_pol)
+        nsl = np.zeros(host.n_pol, np.int32)
+        _check(self.lib.elp_ring_chain_stats(self.h, l2i.ctypes.data, host.n_pol, host.N, rg2.ctypes.data,
+                                             nsl.ctypes.data), self.lib)
+        return rg2, nsl
 
     def ring_tuv(self, host):
         """The tuv file text of the device lattice (bytes), from the device-side recovery."""

@@ -48,6 +48,18 @@ def test_device_tuv_equals_host_writer(tmp_path, N, L, seed, n_time):
     eng.close()
 
 
+def test_device_chain_stats_match_statistics_library():
+    from elp.stats import Series
+    h, eng = _run(1000, 96, 8, 40)
+    rg2, nsl = eng.ring_stats(h)
+    h.set_lattice(eng.download())
+    s = Series(True, h.L, h.n_pol, h.N, 1)
+    s.add_host(h)
+    assert np.allclose(rg2, s.rg2(0), rtol=1e-12, atol=1e-9)
+    assert np.isclose(nsl.sum() / (h.n_pol * h.N), s.statics(0)["slrat"])
+    eng.close()
+
+
 def test_device_recovery_rejects_a_damaged_lattice():
     h, eng = _run(100, 32, 6, 5)
     lat = eng.download()
