@@ -76,7 +76,7 @@ def bench_batch(args, tile):
     import elp
     import numpy as np
     dense = args.sublattice == "dense"
-    n_iter = args.n_iter or (13 if dense else 51)
+    n_iter = args.n_iter if args.n_iter is not None else (13 if dense else 51)
     Ls, Ns = [32, 48, 64, 96], [50, 200, 1000, 2000]
     hosts = [elp.RingHost(N=Ns[(r // 4) % 4], L=Ls[r % 4], density=args.density, seed=r, box_unit=16)
              for r in range(64)]
@@ -304,7 +304,7 @@ def main():
     dense = args.sublattice == "dense"
     ring = args.chain == "ring"
     substeps = 4 if ring else 2
-    n_iter = args.n_iter or ((13 if dense else 51) if ring else (16 if dense else 64))
+    n_iter = args.n_iter if args.n_iter is not None else ((13 if dense else 51) if ring else (16 if dense else 64))
     N = args.N or (1000 if ring else 500)
     host = (elp.RingHost if ring else elp.LinHost)(N=N, L=args.L, density=args.density, seed=0)
     eng = elp.Engine("ring" if ring else "linear", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC,

@@ -583,10 +583,15 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
             if (tid < 64) {
                 lp_softmax(S.Z[tid], M.pS[tid]);
                 const uint32_t um = (s < 2) ? word_of(r0, 2 * s) : word_of(r1, 2 * (s - 2));
-                float u = (float)(um >> 8) * (1.f / 16777216.f);
-                int m = LP_NMOVE - 1;
+                // u in [0, sum of the probabilities as summed here): the loop
+                // always stops on a move of nonzero probability
+                float tot = 0;
+                for (int k = 0; k < LP_NMOVE; k++) tot += M.pS[tid][k];
+                float u = (float)(um >> 8) * (1.f / 16777216.f) * tot;
+                int m = 0;
                 for (int k = 0; k < LP_NMOVE; k++) {
-                    if (u < M.pS[tid][k]) { m = k; break; }
+                    if (M.pS[tid][k] > 0.f) m = k;
+                    if (u < M.pS[tid][k]) break;
                     u -= M.pS[tid][k];
                 }
                 int rv = -1, ch[3] = {PC, PC, PC};
