@@ -30,7 +30,7 @@ struct Cand {
 const Cand kCands[] = {
     {16, 16, 16, ELP_SUBLAT_CUBIC, 64},  {16, 16, 16, ELP_SUBLAT_DENSE, 64},
     {16, 16, 16, ELP_SUBLAT_DENSE, 128}, {16, 16, 16, ELP_SUBLAT_DENSE, 256},
-    {32, 32, 32, ELP_SUBLAT_CUBIC, 256}, {32, 32, 32, ELP_SUBLAT_DENSE, 512},
+    {16, 16, 8, ELP_SUBLAT_DENSE, 128},  {32, 32, 32, ELP_SUBLAT_CUBIC, 256}, {32, 32, 32, ELP_SUBLAT_DENSE, 512},
 };
 
 int default_iters(int kind, int sublattice) {

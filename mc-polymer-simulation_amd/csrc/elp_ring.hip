@@ -369,7 +369,7 @@ __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__
 // P-indexed LDS only for 16^3 tiles (16 KiB); returns whether it launched
 template <int TT, int TU, int TV, bool DENSE, int TH>
 static bool launch_padded(int variant, int nblocks, hipStream_t stream, uint8_t* glat, const RingLaunch& p) {
-    if constexpr (TT == 16 && TU == 16 && TV == 16) {
+    if constexpr (TT == 16 && TU == 16 && TV <= 16) {
         if (variant == 3) {
             hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 1>), dim3(nblocks), dim3(TH), 0, stream, glat, p);
             return true;
@@ -415,6 +415,7 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
     ELP_RING_CASE(16, 16, 32, false, 128)
     ELP_RING_CASE(16, 16, 32, true, 128)
     ELP_RING_CASE(16, 16, 32, true, 256)
+    ELP_RING_CASE(16, 16, 8, true, 128)
     return -1;
 }
 
@@ -422,7 +423,7 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
 #define ELP_RING_BATCH_CASE(TT, TU, TV, DENSE, TH)                                                         \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
-        constexpr int PAD = (TT == 16 && TU == 16 && TV == 16) ? 1 : 0;                                    \
+        constexpr int PAD = (TT == 16 && TU == 16 && TV <= 16) ? 1 : 0;                                    \
         hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD>), dim3(nblocks), dim3(TH), 0, stream, \
                            gbase, b);                                                                      \
         return (int)hipGetLastError();                                                                     \
@@ -434,6 +435,7 @@ int ring_batch_launch(int tt, int tu, int tv, bool dense, int threads, bool prob
     ELP_RING_BATCH_CASE(16, 16, 16, true, 64)
     ELP_RING_BATCH_CASE(16, 16, 16, true, 128)
     ELP_RING_BATCH_CASE(16, 16, 16, true, 256)
+    ELP_RING_BATCH_CASE(16, 16, 8, true, 128)
     ELP_RING_BATCH_CASE(32, 32, 32, false, 256)
     ELP_RING_BATCH_CASE(32, 32, 32, true, 512)
     return -1;

@@ -61,7 +61,8 @@ def _run_batch_vs_solo(kind, host_cls, spec, tile, sublat, n_iter, runs, replay)
 
 
 @pytest.mark.parametrize("tile,sublat,n_iter", [((16, 16, 16), elp.SUBLAT_DENSE, 13),
-                                                ((16, 16, 16), elp.SUBLAT_CUBIC, 51)])
+                                                ((16, 16, 16), elp.SUBLAT_CUBIC, 51),
+                                                ((16, 16, 8), elp.SUBLAT_DENSE, 13)])
 def test_ring_batch_bit_exact_vs_solo(tile, sublat, n_iter):
     _say(f"[ring batch] tile={tile} sublat={sublat}")
     spec = [(50, 32, 0), (200, 48, 1), (100, (64, 32, 48), 2), (1000, 32, 3)]

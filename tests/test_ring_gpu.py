@@ -16,6 +16,7 @@ SCHEDULES = [  # tile, sublattice, iterations per time unit, threads
     ((16, 16, 16), elp.SUBLAT_DENSE, 13),
     ((32, 32, 16), elp.SUBLAT_DENSE, 13),
     ((16, 16, 32), elp.SUBLAT_CUBIC, 51),
+    ((16, 16, 8), elp.SUBLAT_DENSE, 13),
 ]
 
 
