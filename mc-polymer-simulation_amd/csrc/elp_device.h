@@ -80,9 +80,15 @@ struct PTile {
     static constexpr uint32_t MASK = (uint32_t)(TT - 1) | ((uint32_t)(TU - 1) << SU) | ((uint32_t)(TV - 1) << SV);
     static constexpr int NSITE = PAD ? (1 << (SV + LV)) : TT * TU * TV;
     __device__ static __forceinline__ uint32_t pack(uint32_t t, uint32_t u, uint32_t v) { return t | (u << SU) | (v << SV); }
+    // Bond code b -> offset of P: bits 0-2 of b spread to the field origins
+    // 0, SU, SV by one 24-bit multiply (the three shifted copies of b & 7 sit
+    // at bits [0,2], [SU-1,SU+1], [SV-2,SV], disjoint since LT, LU >= 3, so
+    // no carries), then (1,1,1) subtracted when bit 3 is set.
+    static constexpr uint32_t SPREAD = 1u | (1u << (SU - 1)) | (1u << (SV - 2));
+    static_assert(LT >= 3 && LU >= 3, "spread multiply needs tile edges >= 8");
     __device__ static __forceinline__ uint32_t delta(uint32_t b) {
-        const uint32_t d = (b & 1u) | ((b & 2u) << (SU - 1)) | ((b & 4u) << (SV - 2));
-        return (b & 8u) ? d - ONES : d;
+        const uint32_t d = __umul24(b & 7u, SPREAD) & ONES;
+        return d - (ONES & (uint32_t)((int32_t)(b << 28) >> 31));
     }
     __device__ static __forceinline__ bool out(uint32_t P) { return (P & GUARD) != 0u; }
     __device__ static __forceinline__ uint32_t addr(uint32_t P) {

@@ -411,9 +411,13 @@ class _Host:
     libname = None
     dtype = None
 
-    def __init__(self, N=None, L=None, density=1.2, seed=0, path=None, box_unit=0):
+    def __init__(self, N=None, L=None, density=1.2, seed=0, path=None, box_unit=0, init="reference",
+                 site_fraction=0.0):
         """box_unit: box edges are truncated to multiples of it (0 = 32, the
-        reference's rule; 16 allows boxes such as 48 for 16^3-tile engines)."""
+        reference's rule; 16 allows boxes such as 48 for 16^3-tile engines).
+        init: "reference" (the reference's compact chains) or, for linear
+        chains, "saw" (random self-avoiding walks covering site_fraction of
+        the box; 0 takes the library default)."""
         self.lib = _load(self.libname)
         p = self.prefix
         lib = self.lib
@@ -437,8 +441,19 @@ class _Host:
             getattr(lib, p + "seed").restype = C.c_uint32
             getattr(lib, p + "seed").argtypes = [C.c_void_p]
             lib._host_typed = True
+        if init not in ("reference", "saw"):
+            raise ValueError(f"unknown init {init!r}")
         if path is not None:
             h = getattr(lib, p + "from_file")(os.fsencode(path), int(seed))
+        elif init == "saw":
+            if not hasattr(lib, p + "new_saw"):
+                raise ElpError("the self-avoiding-walk start exists for linear chains only")
+            f = getattr(lib, p + "new_saw")
+            f.restype = C.c_void_p
+            f.argtypes = [C.c_int] * 4 + [C.c_double, C.c_uint32, C.c_int, C.c_double]
+            Ls = L if hasattr(L, "__len__") else (L, L, L)
+            h = f(int(N), int(Ls[0]), int(Ls[1]), int(Ls[2]), float(density), int(seed), int(box_unit),
+                  float(site_fraction))
         else:
             Ls = L if hasattr(L, "__len__") else (L, L, L)
             h = getattr(lib, p + "new_unit")(int(N), int(Ls[0]), int(Ls[1]), int(Ls[2]), float(density), int(seed),

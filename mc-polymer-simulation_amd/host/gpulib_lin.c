@@ -103,6 +103,7 @@ int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULib
 
 /* ---------------- handle API (bench.py, tests, other drivers) ---------------- */
 typedef struct elp_lin_host { SimProperties sp; SimState ss; } elp_lin_host;
+void elp_lin_host_free(elp_lin_host* h);
 
 /* Fresh start with box edges truncated to multiples of box_unit (0: 32, as the
    reference; a smaller unit, e.g. 16, allows boxes such as 48 for engines with
@@ -117,12 +118,31 @@ elp_lin_host* elp_lin_host_new_unit(int N, int LT, int LU, int LV, double densit
     SimStateInit(&h->ss, &h->sp);
     GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
     RedistribSL(&h->ss, &h->sp);
-    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { elp_lin_host_free(h); return NULL; }
     return h;
 }
 
 elp_lin_host* elp_lin_host_new(int N, int LT, int LU, int LV, double density, uint32_t seed) {
     return elp_lin_host_new_unit(N, LT, LU, LV, density, seed, 0);
+}
+
+/* Fresh start whose chains are random self-avoiding walks covering about
+   site_fraction of the box (lin_saw.c); site_fraction <= 0 takes
+   ELP_SAW_SITE_FRACTION.  Stored length is then spread by RedistribSL. */
+elp_lin_host* elp_lin_host_new_saw(int N, int LT, int LU, int LV, double density, uint32_t seed, int box_unit,
+                                   double site_fraction) {
+    if (box_unit < 0 || (box_unit && box_unit % 4)) return NULL;
+    elp_lin_host* h = (elp_lin_host*)calloc(1, sizeof *h);
+    SimPropDefaultInit(&h->sp);
+    h->sp.polLength = N; h->sp.density = density; h->sp.seed = seed; h->sp.nDevices = 1;
+    h->sp.boxUnit = box_unit;
+    h->sp.sawSiteFraction = site_fraction > 0 ? site_fraction : ELP_SAW_SITE_FRACTION;
+    SetBoxDimension(&h->sp, LT, LU, LV);
+    SimStateInit(&h->ss, &h->sp);
+    if (BuildSAWPolymers(&h->sp, &h->ss, N, density, h->sp.sawSiteFraction)) { elp_lin_host_free(h); return NULL; }
+    RedistribSL(&h->ss, &h->sp);
+    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { elp_lin_host_free(h); return NULL; }
+    return h;
 }
 
 elp_lin_host* elp_lin_host_from_file(const char* path, uint32_t seed) {

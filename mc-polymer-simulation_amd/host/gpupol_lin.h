@@ -74,6 +74,7 @@ typedef struct SimProperties {
     uint maxPolLength;
     double density;
     int boxUnit;       /* box edges are truncated to multiples of this; 0 = the reference's 32 */
+    double sawSiteFraction; /* > 0: fresh starts grow self-avoiding walks covering this site fraction (lin_saw.c) */
 } SimProperties;
 
 typedef struct GPULibContext { int devIds[8]; } GPULibContext;
@@ -94,6 +95,8 @@ void SimStateFree(SimState* ss);
 void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, double density);
 void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, int length, uint* bonds);
 void RedistribSL(SimState* ss, SimProperties* sp);
+#define ELP_SAW_SITE_FRACTION 0.7
+int  BuildSAWPolymers(SimProperties* sp, SimState* ss, int polLength, double density, double siteFraction);
 void SetBondVecs(SimState* ss, SimProperties* sp, uint t, uint u, uint v, uint* bonds, uint polSize, uint* labels);
 int  GetAllRingPolymers(SimProperties* sp, SimState* ss);
 int  UpdatePolymerWithLabels(SimState* ss);

@@ -5,9 +5,14 @@
  * As in the reference, FAST_EQ reshuffles the stored length of the host copy
  * before each interval (gpupol/gpupol.c:54); the device lattice is not
  * reloaded, so it only advances the host random stream.
+ * ELP_LIN_INIT=saw[:FRACTION] makes a fresh start grow the chains as random
+ * self-avoiding walks covering FRACTION of the sites (default
+ * ELP_SAW_SITE_FRACTION) instead of the reference's folded chains; resumed
+ * runs read their tuv file as before.
  */
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <sys/time.h>
 #include "gpupol_lin.h"
 
@@ -36,6 +41,8 @@ int main(int argc, char** argv) {
         fprintf(stderr, "usage: %s NMONO TIME SEED DIR [DENSITY FAST_EQ INTERVAL LT LU LV EQUILIBRATED DBL_STEP]\n", argv[0]);
         return 2;
     }
+    const char* init = getenv("ELP_LIN_INIT");
+    if (init && !strncmp(init, "saw", 3)) sp.sawSiteFraction = init[3] == ':' ? atof(init + 4) : ELP_SAW_SITE_FRACTION;
     if (!sp.LT) sp.LT = 32;
     if (!sp.LU) sp.LU = sp.LT;
     if (!sp.LV) sp.LV = sp.LT;

@@ -518,7 +518,11 @@ void LoadPolymers(SimProperties* sp, SimState* ss, GPUDeviceState* devices, GPUL
     } else {
         SetBoxDimension(sp, (int)sp->LT, (int)sp->LU, (int)sp->LV);
         SimStateInit(ss, sp);
-        GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+        if (sp->sawSiteFraction > 0) {
+            if (BuildSAWPolymers(sp, ss, sp->polLength, sp->density, sp->sawSiteFraction)) exit(192);
+        } else {
+            GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+        }
         RedistribSL(ss, sp);
         for (int i = 0; i < sp->nDevices; i++) {
             snprintf(file, sizeof file, "%s/t=0_dev=%i.res", sp->dirOut, i);

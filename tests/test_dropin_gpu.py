@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import elp
-from ringcheck import replay_ring
+from ringcheck import replay_lin, replay_ring
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(elp.PKG_DIR, "bin")
@@ -84,3 +84,18 @@ def test_linear_dropin(tmp_path):
     assert len(chains) == 197 and all(c.endswith("f") and len(c) == 200 for c in chains)
     r = elp.LinHost(path=str(d / "t=200_dev=0.res"))
     assert r.n_pol == 197
+
+
+def test_linear_dropin_saw_start(tmp_path):
+    """ELP_LIN_INIT=saw: the executable's fresh start is the handle API's
+    self-avoiding-walk start, and t=100 is its replay."""
+    d = tmp_path / "lsaw"
+    _run("gpupol_cuda_lin", [100, 100, 6, d, 1.2, 0, 100, 32, 32, 32], env=dict(os.environ, ELP_LIN_INIT="saw"))
+    h = elp.LinHost(N=100, L=32, density=1.2, seed=6, init="saw")
+    h.write_tuv(tmp_path / "t0.res")
+    assert filecmp.cmp(d / "t=0_dev=0.res", tmp_path / "t0.res", shallow=False)
+    lat0 = h.lattice()
+    org = h.origins((32, 32, 32), 100)
+    h.set_lattice(replay_lin(lat0, h.L, (32, 32, 32), elp.SUBLAT_CUBIC, 64, org, 0, 6))
+    h.write_tuv(tmp_path / "t100.res")
+    assert filecmp.cmp(d / "t=100_dev=0.res", tmp_path / "t100.res", shallow=False)

@@ -49,3 +49,37 @@ def test_lin_full_size_invariants():
     assert lin_site_invariants(got, h.L) == (5033 * 500, 5033)
     assert np.count_nonzero(got != lat0) > lat0.size // 4
     h.set_lattice(got)
+
+
+@pytest.mark.parametrize("tile,sublat,n_iter", [SCHEDULES[0], SCHEDULES[2]])
+def test_lin_saw_start_bit_exact_vs_replay(tile, sublat, n_iter):
+    print(f"[lin parity, self-avoiding-walk start] tile={tile} sublat={sublat} L=32", flush=True)
+    h = elp.LinHost(N=100, L=32, density=1.2, seed=2, init="saw")
+    lat0 = h.lattice()
+    org = h.origins(tile, 10)
+    eng = elp.Engine("linear", h.L, tile, sublat, n_iter)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    want = replay_lin(lat0, h.L, tile, sublat, n_iter, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > 1000
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
+
+
+def test_lin_saw_start_full_size():
+    """Linear target configuration: L=128, N=500 (5033 chains) grown as random
+    self-avoiding walks, 200 time units on the dense 16^3 schedule."""
+    print("[lin full size, self-avoiding-walk start] L=128 N=500", flush=True)
+    h = elp.LinHost(N=500, L=128, density=1.2, seed=3, init="saw")
+    lat0 = h.lattice()
+    seq0 = lin_label_sequences(lat0, h.L)
+    tile = (16, 16, 16)
+    org = h.origins(tile, 200)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    assert lin_site_invariants(got, h.L) == (5033 * 500, 5033)
+    assert lin_label_sequences(got, h.L) == seq0
+    assert np.count_nonzero(got != lat0) > lat0.size // 4
+    h.set_lattice(got)
