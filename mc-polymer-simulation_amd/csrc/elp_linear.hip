@@ -363,6 +363,8 @@ struct LinMoves {
     using site_t = uint16_t;
     static constexpr int NSUB = 2;
     template <class G>
+    __device__ static __forceinline__ void init(uint16_t*, int) {}
+    template <class G>
     __device__ static __forceinline__ void move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {
         if (s == 0) lin_diffuse_sl2<G>(lat, P);
         else lin_trans_move2<G>(lat, P, r);

@@ -244,15 +244,27 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //     registers.  Accept/reject rules and writes are those of the functions
 //     above.
 // =====================================================================
-template <class G>
+// The ordered bond pair (nb1, nb2) that replaces bond `next` for random bits
+// r & 3: nb1 = trans_first(next, r & 1), nb2 = the bond with nb1 + nb2 = next,
+// swapped when bit 1 is clear.
+__device__ __forceinline__ uint32_t fold_pair(uint32_t next, uint32_t r) {
+    uint32_t nb1 = trans_first(next, r & 1u);
+    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
+    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+    return nb1 | (nb2 << 4);
+}
+
+// PAIRS: read fold_pair from a 64-byte LDS table after the tile
+// (lat[G::NSITE + 4 next + (r & 3)], filled by RingMovesT::init) instead of
+// computing it: one more LDS round trip, about 12 fewer VALU instructions.
+template <class G, bool PAIRS = false>
 __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);
-    uint32_t nb1 = trans_first(next, r & 1u);
-    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
-    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+    const uint32_t pair = PAIRS ? (uint32_t)lat[G::NSITE + ((next << 2) | (r & 3u))] : fold_pair(next, r);
+    const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4;
     const uint32_t Pd = P + G::delta(nb1);
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
     uint32_t ns = lat[na];
@@ -331,9 +343,15 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
 
 // ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
 // (gpupol2/ringpol3.cu:278-292) as the move set of tile_unit.
-struct RingMoves {
+template <bool PAIRS>
+struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
+    static constexpr int EXTRA = PAIRS ? 64 : 0;   // LDS bytes after the tile
+    template <class G>
+    __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
+        if (PAIRS && tid < 64) lat[G::NSITE + tid] = (uint8_t)fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
+    }
     template <class G>
     __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
         if (rules) {
@@ -342,45 +360,58 @@ struct RingMoves {
             // proportions of the reference sequence), so each substep is a
             // symmetric kernel and preserves the uniform measure by itself
             const uint32_t ty = r >> 6;
-            if (ty < 2u) trans_forw2<G>(lat, P, r);
+            if (ty < 2u) trans_forw2<G, PAIRS>(lat, P, r);
             else if (ty == 2u) diffuse_sl2<G>(lat, P);
             else trans_back2<G>(lat, P, r, rules);
             return;
         }
         if (s == 1) diffuse_sl2<G>(lat, P);
         else if (s == 3) trans_back2<G>(lat, P, r, rules);
-        else trans_forw2<G>(lat, P, r);
+        else trans_forw2<G, PAIRS>(lat, P, r);
     }
 };
+using RingMoves = RingMovesT<false>;
 
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p) {
-    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE];
-    tile_unit<RingMoves, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p), xcd_tile(blockIdx.x, gridDim.x));
+    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
+    tile_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p),
+                                                                  xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // One time unit of every replica of a batch (one workgroup per tile).
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__ gbase, BatchLaunch b) {
-    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE];
-    batch_unit<RingMoves, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
+    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
+    batch_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
 }
 
-// P-indexed LDS only for 16^3 tiles (16 KiB); returns whether it launched
+// ring_time_unit2 for variants 2-4: P-indexed (padded) LDS only for
+// 16x16x{8,16} tiles (at most 16 KiB), compact otherwise; variant 4 adds the
+// LDS bond-pair table of TransForw.
 template <int TT, int TU, int TV, bool DENSE, int TH>
-static bool launch_padded(int variant, int nblocks, hipStream_t stream, uint8_t* glat, const RingLaunch& p) {
+static void launch2(int variant, int nblocks, hipStream_t stream, uint8_t* glat, const RingLaunch& p) {
     if constexpr (TT == 16 && TU == 16 && TV <= 16) {
-        if (variant == 3) {
+        if (variant == 4)
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 1, true>), dim3(nblocks), dim3(TH), 0, stream,
+                               glat, p);
+        else if (variant == 3)
             hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 1>), dim3(nblocks), dim3(TH), 0, stream, glat, p);
-            return true;
-        }
+        else
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 0>), dim3(nblocks), dim3(TH), 0, stream, glat, p);
+    } else {
+        if (variant == 4)
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 0, true>), dim3(nblocks), dim3(TH), 0, stream,
+                               glat, p);
+        else
+            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 0>), dim3(nblocks), dim3(TH), 0, stream, glat, p);
     }
-    return false;
 }
 
 // ---- dispatch over the compiled schedules ----
 // variant 1: ring_time_unit (cell-major LDS); 2: ring_time_unit2 compact;
-// 3: ring_time_unit2 with P-indexed (padded) LDS.
+// 3: ring_time_unit2 with P-indexed (padded) LDS where the tile allows it;
+// 4 (default): as 3 with the LDS bond-pair table of TransForw.
 #define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                              \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
@@ -388,18 +419,17 @@ static bool launch_padded(int variant, int nblocks, hipStream_t stream, uint8_t*
         if (variant == 1)                                                                                  \
             hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, stream, \
                                glat, p);                                                                   \
-        else if (!launch_padded<TT, TU, TV, DENSE, TH>(variant, nblocks, stream, glat, p))               \
-            hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 0>), dim3(nblocks), dim3(TH), 0,    \
-                               stream, glat, p);                                                           \
+        else                                                                                               \
+            launch2<TT, TU, TV, DENSE, TH>(variant, nblocks, stream, glat, p);                             \
         return (int)hipGetLastError();                                                                     \
     }
 
 int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                 uint8_t* glat, const RingLaunch& p) {
-    // default: P-indexed LDS for 16^3 tiles (fastest measured), compact otherwise
+    // default: P-indexed LDS where the tile allows it, pair table (fastest measured)
     static const int variant = [] {
         const char* env = getenv("ELP_RING_VARIANT");
-        return env ? atoi(env) : 3;
+        return env ? atoi(env) : 4;
     }();
     ELP_RING_CASE(16, 16, 16, false, 64)
     ELP_RING_CASE(16, 16, 16, true, 64)
@@ -424,13 +454,22 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
         constexpr int PAD = (TT == 16 && TU == 16 && TV <= 16) ? 1 : 0;                                    \
-        hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD>), dim3(nblocks), dim3(TH), 0, stream, \
-                           gbase, b);                                                                      \
+        if (batch_pairs)                                                                                   \
+            hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD, true>), dim3(nblocks), dim3(TH), 0, \
+                               stream, gbase, b);                                                          \
+        else                                                                                               \
+            hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD>), dim3(nblocks), dim3(TH), 0, stream, \
+                               gbase, b);                                                                  \
         return (int)hipGetLastError();                                                                     \
     }
 
 int ring_batch_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                       uint8_t* gbase, const BatchLaunch& b) {
+    // LDS bond-pair table of TransForw unless ELP_RING_PAIRS=0 (measured +9 %)
+    static const bool batch_pairs = [] {
+        const char* env = getenv("ELP_RING_PAIRS");
+        return !env || atoi(env) != 0;
+    }();
     ELP_RING_BATCH_CASE(16, 16, 16, false, 64)
     ELP_RING_BATCH_CASE(16, 16, 16, true, 64)
     ELP_RING_BATCH_CASE(16, 16, 16, true, 128)

@@ -6,7 +6,8 @@
 // set's substeps with a workgroup barrier after each substep, and writes the
 // tile back in place.  The move set M supplies the site type, the number of
 // substeps per iteration and the move of substep s:
-//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules).
+//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules),
+// and fills any LDS tables it keeps after the tile: M::init<G>(lds, tid).
 //
 // Randomness (Philox4x32-10, key = (seed, stream)):
 //   site offsets : counter (tau, tile, 0xFFFFFFFF, iter/4), word iter%4,
@@ -45,6 +46,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     constexpr int NSLOT = NCELL * SPC;
     constexpr int SPT = NSLOT / THREADS;
     static_assert(NSLOT % THREADS == 0 && SPT >= 1, "slots must split evenly over threads");
+    static_assert(THREADS >= 64, "M::init fills its tables with the first 64 threads");
 
     const int tid = threadIdx.x;
     const int kt = (int)(tileId % (uint32_t)p.ntile_t);
@@ -73,6 +75,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
         j3[k] = 3u * j;
         j2[k] = 2u * j;
     }
+    M::template init<G>(tile, tid);
     __syncthreads();
 
     for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
@@ -82,7 +85,10 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
         for (int k = 0; k < SPT; k++)
             sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
         const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
-        for (int j = 0; j < nj; j++) {
+        // unrolled, so that word j of each Philox output is a fixed register
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= nj) break;
             const uint32_t offw = word_of(ow, j);
 #pragma unroll
             for (int s = 0; s < M::NSUB; s++) {
