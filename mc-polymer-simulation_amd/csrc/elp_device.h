@@ -63,6 +63,21 @@ __device__ __forceinline__ uint32_t trans_first(uint32_t next, uint32_t k) {
     return (uint32_t)(w >> (4u * (e & 15u))) & 0xfu;
 }
 
+// The ordered bond pair (nb1, nb2) that replaces bond `next` for random bits
+// r & 3: nb1 = trans_first(next, r & 1), nb2 = the bond with nb1 + nb2 = next,
+// swapped when bit 1 is clear; returned as nb1 | nb2 << 4.  The move kernels
+// read it from a 64-entry LDS table, kFoldPairs bytes after the tile.
+__device__ __forceinline__ uint32_t fold_pair(uint32_t next, uint32_t r) {
+    uint32_t nb1 = trans_first(next, r & 1u);
+    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
+    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
+    return nb1 | (nb2 << 4);
+}
+constexpr int kFoldPairs = 64;
+__device__ __forceinline__ void fill_fold_pairs(uint8_t* table, int tid) {
+    if (tid < kFoldPairs) table[tid] = (uint8_t)fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
+}
+
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
 // Packed tile coordinates P = t | u<<SU | v<<SV with one guard bit per field

@@ -276,6 +276,14 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
     lat[na] = (uint16_t)x;
 }
 
+// The two sites a case reads are X2 = P + delta(c2) and X1 = base + delta(c1):
+//   split   (stored length, bondS != 0): X2 = B (bond partner), X1 = N
+//           (inserted site, from B when the bond is prev, else from P);
+//   grow    (stored length, bondS == 0): X1 = X2 = Gr = P + delta(nb);
+//   retract (no stored length, chain end): X1 = X2 = E;
+//   fold    (no stored length, interior): X1 = Pp, X2 = Pn.
+// Only the two offsets of the taken case are computed, and the bond pair of
+// the split comes from the LDS table after the tile (fold_pair).
 template <class G>
 __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
@@ -286,20 +294,16 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     const uint32_t pbS = (r >> 2) & 1u;
     const uint32_t bondS = pbS ? prev : next;
     const uint32_t nb = (r >> 3) & 0xfu;
-    uint32_t nPrev = trans_first(bondS, r & 1u);
-    uint32_t nNext = bond_add((~nPrev) & 0xfu, bondS);
-    if (!(r & 2u)) { const uint32_t x = nPrev; nPrev = nNext; nNext = x; }
-    const uint32_t B = P + G::delta(pbS ? (bondS ^ 0xfu) : bondS);   // bond partner
-    const uint32_t N = (pbS ? B : P) + G::delta(nPrev);               // inserted site
-    const uint32_t Gr = P + G::delta(nb);                             // growth site
+    const uint32_t pair = reinterpret_cast<const uint8_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
+    const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
     // cases without stored length: retract an end, or fold an interior monomer
     const bool end = (next == 0u) || (prev == 0u);
     const uint32_t bondE = (next == 0u) ? (prev ^ 0xfu) : next;
-    const uint32_t E = P + G::delta(bondE);
-    const uint32_t Pp = P + G::delta((~prev) & 0xfu), Pn = P + G::delta(next);
-    uint32_t X1, X2;
-    if (sl) { X1 = bondS ? N : Gr; X2 = bondS ? B : Gr; }
-    else { X1 = end ? E : Pp; X2 = end ? E : Pn; }
+    uint32_t c1, c2;
+    if (sl) { c2 = bondS ? (pbS ? (bondS ^ 0xfu) : bondS) : nb; c1 = bondS ? nPrev : nb; }
+    else { c2 = end ? bondE : next; c1 = end ? bondE : (prev ^ 0xfu); }
+    const uint32_t X2 = P + G::delta(c2);
+    const uint32_t X1 = ((sl && bondS && pbS) ? X2 : P) + G::delta(c1);
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     uint32_t x1 = lat[a1];
     uint32_t x2 = lat[a2];
@@ -307,7 +311,7 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     if (sl) {
         if (bondS == 0u) {   // grow the chain end out of the stored length
             const uint32_t pb = pbS ^ 1u;
-            if (nb == 0u || nb == 3u || nb == 0xcu || nb == 0xfu || G::out(Gr) || (x1 & 0xffu)) return;
+            if (nb == 0u || nb == 3u || nb == 0xcu || nb == 0xfu || G::out(X1) || (x1 & 0xffu)) return;
             const uint32_t ins = pb ? nb : (nb ^ 0xfu);
             const uint32_t lab = (w >> 9) & 3u;
             w = (w & ~(kL0 | kL1 | kSL)) | (((lab >> pb) & 1u) << 9) | (ins << (4 * (pb ^ 1u)));
@@ -316,7 +320,7 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
             lat[a1] = (uint16_t)x1;
             return;
         }
-        if (G::out(B) || G::out(N) || (x1 & 0xffu)) return;   // split the bond through an empty site
+        if (G::out(X2 | X1) || (x1 & 0xffu)) return;   // split the bond through an empty site
         const uint32_t lab = (w >> (9 + pbS)) & 1u;
         x1 |= nNext | (nPrev << 4) | (lab << 9);
         if (pbS) {
@@ -333,7 +337,7 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     }
     if ((r >> 4) & 1u) return;
     if (end) {   // retract a chain end onto its neighbour as stored length
-        if (((r >> 5) & 7u) || G::out(E) || (x1 & kSL)) return;
+        if (((r >> 5) & 7u) || G::out(X1) || (x1 & kSL)) return;
         const uint32_t pb = (next == 0u) ? 1u : 0u;
         const uint32_t nl = (((w >> 9) & 1u) << (pb ^ 1u)) | (((x1 >> 9) & 1u) << pb);
         w &= ~(kL0 | (0xfu << (4 * pb)));
@@ -344,7 +348,7 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     }
     // fold the monomer onto its prev (r bit 0) or next neighbour as stored length
     uint32_t bond;
-    if (G::out(Pp) || G::out(Pn) || !bond_add_valid(next, prev, bond)) return;
+    if (G::out(X1 | X2) || !bond_add_valid(next, prev, bond)) return;
     const uint32_t pb = r & 1u;
     if ((pb ? x1 : x2) & kSL) return;
     const uint32_t lab = (w >> 9) & 1u;
@@ -362,8 +366,11 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
 struct LinMoves {
     using site_t = uint16_t;
     static constexpr int NSUB = 2;
+    static constexpr int EXTRA = kFoldPairs;   // LDS bytes after the tile
     template <class G>
-    __device__ static __forceinline__ void init(uint16_t*, int) {}
+    __device__ static __forceinline__ void init(uint16_t* lat, int tid) {
+        fill_fold_pairs(reinterpret_cast<uint8_t*>(lat + G::NSITE), tid);
+    }
     template <class G>
     __device__ static __forceinline__ void move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {
         if (s == 0) lin_diffuse_sl2<G>(lat, P);
@@ -406,9 +413,10 @@ static int launch_lds(K kernel, size_t lds, int nblocks, int threads, hipStream_
         if constexpr (TT == 16 && TU == 16 && TV == 16)                                                        \
             if (variant == 3)                                                                                  \
                 return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, 1>,                                    \
-                                  sizeof(uint16_t) * PTile<TT, TU, TV, 1>::NSITE, nblocks, TH, stream, glat, p); \
-        return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, 0>, sizeof(uint16_t) * TT * TU * TV, nblocks, TH, \
-                          stream, glat, p);                                                                    \
+                                  sizeof(uint16_t) * PTile<TT, TU, TV, 1>::NSITE + LinMoves::EXTRA, nblocks, TH, \
+                                  stream, glat, p);                                                            \
+        return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, 0>,                                            \
+                          sizeof(uint16_t) * TT * TU * TV + LinMoves::EXTRA, nblocks, TH, stream, glat, p);    \
     }
 
 int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
@@ -437,7 +445,7 @@ int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int 
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                         \
         if (probe) return 0;                                                                           \
         constexpr int PAD = (TT == 16 && TU == 16 && TV == 16) ? 1 : 0;                                \
-        constexpr size_t lds = sizeof(uint16_t) * PTile<TT, TU, TV, PAD>::NSITE;                       \
+        constexpr size_t lds = sizeof(uint16_t) * PTile<TT, TU, TV, PAD>::NSITE + LinMoves::EXTRA;     \
         auto kernel = lin_batch_unit<TT, TU, TV, DENSE, TH, PAD>;                                      \
         if (lds > 65536) {                                                                             \
             hipError_t e = hipFuncSetAttribute((const void*)kernel,                                    \

@@ -244,16 +244,6 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //     registers.  Accept/reject rules and writes are those of the functions
 //     above.
 // =====================================================================
-// The ordered bond pair (nb1, nb2) that replaces bond `next` for random bits
-// r & 3: nb1 = trans_first(next, r & 1), nb2 = the bond with nb1 + nb2 = next,
-// swapped when bit 1 is clear.
-__device__ __forceinline__ uint32_t fold_pair(uint32_t next, uint32_t r) {
-    uint32_t nb1 = trans_first(next, r & 1u);
-    uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
-    if (!(r & 2u)) { const uint32_t x = nb1; nb1 = nb2; nb2 = x; }
-    return nb1 | (nb2 << 4);
-}
-
 // PAIRS: read fold_pair from a 64-byte LDS table after the tile
 // (lat[G::NSITE + 4 next + (r & 3)], filled by RingMovesT::init) instead of
 // computing it: one more LDS round trip, about 12 fewer VALU instructions.
@@ -347,10 +337,10 @@ template <bool PAIRS>
 struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
-    static constexpr int EXTRA = PAIRS ? 64 : 0;   // LDS bytes after the tile
+    static constexpr int EXTRA = PAIRS ? kFoldPairs : 0;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
-        if (PAIRS && tid < 64) lat[G::NSITE + tid] = (uint8_t)fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
+        if (PAIRS) fill_fold_pairs(lat + G::NSITE, tid);
     }
     template <class G>
     __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
