@@ -238,11 +238,13 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //   * "outside the tile" is one AND with the guard bits,
 //   * the LDS tile is plain t-fastest (PAD=0: index rebuilt from P in three
 //     bit-field operations) or indexed by P itself (PAD=1, 16^3 tiles only),
-//   * every LDS read a move needs is issued before the accept/reject test,
-//     so TransForw and DiffuseSL wait for two dependent LDS round trips and
-//     TransBack for three, and the trans table is a 128-bit constant in
-//     registers.  Accept/reject rules and writes are those of the functions
-//     above.
+//   * every LDS read a move needs is issued before the accept/reject test;
+//     DiffuseSL waits for two dependent LDS round trips, TransForw for three
+//     (site, bond-pair table, target) and TransBack for four (site, source,
+//     combine table, target): the two small LDS tables after the tile replace
+//     about 20 VALU instructions per iteration, which measured faster than
+//     the shorter chains.  Accept/reject rules and writes are those of the
+//     functions above.
 // =====================================================================
 // PAIRS: read fold_pair from a 64-byte LDS table after the tile
 // (lat[G::NSITE + 4 next + (r & 3)], filled by RingMovesT::init) instead of
@@ -280,7 +282,17 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
 // (reference rate 1) is accepted with probability 1/4 (r bits 1-2 zero), the
 // rate of the TransForw that undoes it, so every move and its reverse have
 // equal probabilities (with the per-slot move choice of RingMoves::move).
-template <class G>
+// Bond next + srcNext as one bond: bits 0-3 the combined bond, bit 4 set when
+// it is a bond (bond_add_valid).  TABLE: read it from a 256-entry LDS table
+// after the fold pairs (lat[G::NSITE + kFoldPairs + 16 next + srcNext]).
+__device__ __forceinline__ uint32_t combine_code(uint32_t next, uint32_t srcNext) {
+    uint32_t c;
+    const bool ok = bond_add_valid(next, srcNext, c);
+    return bond_add(next, srcNext) | (ok ? 0x10u : 0u);
+}
+constexpr int kCombine = 256;
+
+template <class G, bool TABLE = false>
 __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
@@ -289,13 +301,13 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t sa = G::addr(Ps);
     const uint32_t ss = lat[sa];
     const uint32_t srcNext = ss & 0xfu;
-    const bool combine = !((next | srcNext) != 0xfu && (next & srcNext));
-    const uint32_t newNext = bond_add(next, srcNext);
+    const uint32_t cc = TABLE ? (uint32_t)lat[G::NSITE + kFoldPairs + ((next << 4) | srcNext)] : combine_code(next, srcNext);
+    const uint32_t newNext = cc & 0xfu;
     const uint32_t Pw = P + G::delta(newNext);
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
-    if (!s || G::out(Ps | Pw) || (ss & 0x40u) || !combine || newNext == 0x3u || newNext == 0xcu || (sl && wsl)) return;
+    if (!s || G::out(Ps | Pw) || (ss & 0x40u) || !(cc & 0x10u) || (sl && wsl)) return;
     if (rules && (sl | wsl) && (r & 6u)) return;
     const bool moveFirst = (!sl && !wsl) ? (r & 1u) : (sl == 0);
     const uint32_t srcLabel = ss & 0x30u;
@@ -337,10 +349,14 @@ template <bool PAIRS>
 struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
-    static constexpr int EXTRA = PAIRS ? kFoldPairs : 0;   // LDS bytes after the tile
+    static constexpr int EXTRA = PAIRS ? kFoldPairs + kCombine : 0;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
-        if (PAIRS) fill_fold_pairs(lat + G::NSITE, tid);
+        if (!PAIRS) return;
+        fill_fold_pairs(lat + G::NSITE, tid);
+        if (tid < 64)
+            for (int i = tid; i < kCombine; i += 64)
+                lat[G::NSITE + kFoldPairs + i] = (uint8_t)combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
     }
     template <class G>
     __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
@@ -352,11 +368,11 @@ struct RingMovesT {
             const uint32_t ty = r >> 6;
             if (ty < 2u) trans_forw2<G, PAIRS>(lat, P, r);
             else if (ty == 2u) diffuse_sl2<G>(lat, P);
-            else trans_back2<G>(lat, P, r, rules);
+            else trans_back2<G, PAIRS>(lat, P, r, rules);
             return;
         }
         if (s == 1) diffuse_sl2<G>(lat, P);
-        else if (s == 3) trans_back2<G>(lat, P, r, rules);
+        else if (s == 3) trans_back2<G, PAIRS>(lat, P, r, rules);
         else trans_forw2<G, PAIRS>(lat, P, r);
     }
 };
