@@ -214,6 +214,21 @@ int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len,
 int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, double* rg2,
                          int32_t* n_sl);
 
+/* One frame of the statistics of analysis/mode_compute.c reduced on the
+   device, per polymer id c (same walk as elp_ring_chains):
+     start3[3c..]  ring start site (t,u,v) in the box,
+     sum3[3c..]    sum over its monomers of their unwrapped (t,u,v) relative
+                   to the start (the chain's centre of mass is start + sum/n),
+     mid3[3c..]    unwrapped (t,u,v) of monomer n/2 relative to the start,
+     rg2[c], n_sl[c] as elp_ring_chain_stats (either may be NULL),
+   and contacts[g], g < max_len: number of monomer pairs g apart along a ring
+   on one site or on nearest-neighbour sites in the ring's own unwrapped frame,
+   summed over rings (ContactProbability, :429-504; may be NULL).  Rings of up
+   to 8192 monomers.  libelp_stats turns a series of these frames into the
+   statics and mean-square displacements (elp_series_new_summary). */
+int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                           int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
+
 /*
  * Learned proposal for ring equilibration (no reference counterpart).
  * A small network (144 int8 features of a site's 5^3 neighbourhood and bond

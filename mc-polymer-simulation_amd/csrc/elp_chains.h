@@ -13,5 +13,12 @@ int chains_launch_walk(const uint8_t* lat, int LT, int LU, int LV, const uint32_
                        int32_t* line_len, double* rg2, int32_t* nsl, uint32_t* err, hipStream_t st);
 int chains_launch_text(const uint8_t* bonds, const int32_t* lens, const int32_t* start3, const uint64_t* offset,
                        int n_pol, int max_len, char* text, hipStream_t st);
+// Per ring: sum of its monomer positions and its middle monomer relative to
+// the start; contacts[g] += pairs g apart that touch.  n_max = longest ring
+// (<= kSummaryMaxLen: the LDS image is summary_lds_bytes(n_max)).
+constexpr int kSummaryMaxLen = 8192;
+constexpr size_t summary_lds_bytes(int n_max) { return 32 + 4 * 3 * 256 + 16 * (size_t)n_max; }
+int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int64_t* sum3,
+                          int32_t* mid3, unsigned long long* contacts, hipStream_t st);
 
 }  // namespace elp

@@ -149,6 +149,108 @@ __global__ void ring_text_kernel(const uint8_t* __restrict__ bonds, const int32_
     *p = '\n';
 }
 
+// (dt,du,dv) in {-1,0,1}^3 -> bit (dt+1) + 3(du+1) + 9(dv+1): set for the
+// same site and the 12 FCC neighbours (the displacements of the bond codes).
+constexpr uint32_t contact_mask() {
+    uint32_t m = 1u << 13;
+    for (uint32_t b = 1; b < 15; b++) {
+        if (b == 3u || b == 12u) continue;
+        const int w = (int)(b >> 3);
+        const int dt = (int)(b & 1u) - w, du = (int)((b >> 1) & 1u) - w, dv = (int)((b >> 2) & 1u) - w;
+        m |= 1u << ((dt + 1) + 3 * (du + 1) + 9 * (dv + 1));
+    }
+    return m;
+}
+constexpr uint32_t kContactMask = contact_mask();
+constexpr int kSumThreads = 256;
+
+// One workgroup per ring (in id order): unwrapped (t,u,v) of every monomer
+// relative to the ring start (a chunked prefix sum of the bond string in
+// LDS), their sum, the middle monomer (index n/2) and the contact histogram:
+// pairs i < j on one site or on nearest-neighbour sites in the ring's own
+// unwrapped frame, counted at j - i (analysis/mode_compute.c:429-504).
+// Dynamic LDS: 3 int64 sums, 3 x 256 chunk partials, then x, y, z, hist of
+// n_max words each.
+__global__ __launch_bounds__(kSumThreads) void ring_summary_kernel(const uint8_t* __restrict__ bonds,
+                                                                   const int32_t* __restrict__ lens, int max_len,
+                                                                   int n_max, int64_t* __restrict__ sum3,
+                                                                   int32_t* __restrict__ mid3,
+                                                                   unsigned long long* __restrict__ contacts) {
+    extern __shared__ unsigned long long smem[];
+    unsigned long long* ssum = smem;                                   // [4]
+    int32_t* part = reinterpret_cast<int32_t*>(smem + 4);             // [3 * kSumThreads]
+    int32_t* px = part + 3 * kSumThreads;                              // [n_max] each
+    int32_t* py = px + n_max;
+    int32_t* pz = py + n_max;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(pz + n_max);
+    const int id = blockIdx.x, tid = threadIdx.x;
+    const int n = lens[id];                                            // 1 <= n <= n_max
+    const uint8_t* b = bonds + (size_t)id * max_len;
+    const int chunk = (n + kSumThreads - 1) / kSumThreads;
+    const int lo = tid * chunk < n ? tid * chunk : n;
+    const int hi = lo + chunk < n ? lo + chunk : n;
+
+    for (int i = tid; i < n; i += kSumThreads) hist[i] = 0u;
+    if (tid < 3) ssum[tid] = 0ull;
+    int st = 0, su = 0, sv = 0;
+    for (int k = lo; k < hi; k++) { const uint32_t c = b[k] & 0xfu; if (c) { st += bdt(c); su += bdu(c); sv += bdv(c); } }
+    part[3 * tid] = st; part[3 * tid + 1] = su; part[3 * tid + 2] = sv;
+    __syncthreads();
+    if (tid == 0) {                                                    // exclusive scan of the chunk sums
+        int at = 0, au = 0, av = 0;
+        for (int k = 0; k < kSumThreads; k++) {
+            const int xt = part[3 * k], xu = part[3 * k + 1], xv = part[3 * k + 2];
+            part[3 * k] = at; part[3 * k + 1] = au; part[3 * k + 2] = av;
+            at += xt; au += xu; av += xv;
+        }
+    }
+    __syncthreads();
+    int t = part[3 * tid], u = part[3 * tid + 1], v = part[3 * tid + 2];
+    long long mt = 0, mu = 0, mv = 0;
+    for (int i = lo; i < hi; i++) {
+        px[i] = t; py[i] = u; pz[i] = v;
+        mt += t; mu += u; mv += v;
+        const uint32_t c = b[i] & 0xfu;
+        if (c) { t += bdt(c); u += bdu(c); v += bdv(c); }
+    }
+    if (hi > lo) {
+        atomicAdd(&ssum[0], (unsigned long long)mt);
+        atomicAdd(&ssum[1], (unsigned long long)mu);
+        atomicAdd(&ssum[2], (unsigned long long)mv);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kSumThreads) {
+        const int ti = px[i] - 1, ui = py[i] - 1, vi = pz[i] - 1;
+        for (int j = i + 1; j < n; j++) {
+            // d = displacement + 1 per axis; a touching pair has every d in {0,1,2}
+            const uint32_t d0 = (uint32_t)(px[j] - ti), d1 = (uint32_t)(py[j] - ui), d2 = (uint32_t)(pz[j] - vi);
+            if (max(d0, max(d1, d2)) > 2u) continue;
+            if ((kContactMask >> (d0 + 3u * d1 + 9u * d2)) & 1u) atomicAdd(&hist[j - i], 1u);
+        }
+    }
+    __syncthreads();
+    for (int g = tid; g < n; g += kSumThreads)
+        if (hist[g]) atomicAdd(&contacts[g], (unsigned long long)hist[g]);
+    if (tid == 0) {
+        sum3[3 * id] = (int64_t)ssum[0]; sum3[3 * id + 1] = (int64_t)ssum[1]; sum3[3 * id + 2] = (int64_t)ssum[2];
+        const int m = n / 2;
+        mid3[3 * id] = px[m]; mid3[3 * id + 1] = py[m]; mid3[3 * id + 2] = pz[m];
+    }
+}
+
+int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int64_t* sum3,
+                          int32_t* mid3, unsigned long long* contacts, hipStream_t st) {
+    const size_t lds = summary_lds_bytes(n_max);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)ring_summary_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(ring_summary_kernel, dim3(n_pol), dim3(kSumThreads), lds, st, bonds, lens, max_len, n_max, sum3,
+                       mid3, contacts);
+    return (int)hipGetLastError();
+}
+
 int chains_launch_heads(const uint8_t* lat, int LT, int LU, int LV, uint32_t* starts, uint32_t cap, uint32_t* count,
                         hipStream_t st) {
     const uint32_t n = (uint32_t)LT * LU * LV;

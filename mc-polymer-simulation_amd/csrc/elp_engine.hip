@@ -853,7 +853,8 @@ int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origin
 // and start site; optionally the tuv text.  Shared by elp_ring_chains and
 // elp_ring_tuv.
 struct ChainBufs {
-    DevBuf l2i, starts, count, bonds, lens, start3, line_len, rg2, nsl, err, offset, text;
+    DevBuf l2i, starts, count, bonds, lens, start3, line_len, rg2, nsl, err, offset, text, sum3, mid3, contacts;
+    int n_max = 0;   // longest ring
 };
 
 static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
@@ -896,8 +897,10 @@ static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max
     HIPCHK(hipStreamSynchronize(e->stream));
     if (err & 1u) return fail(ELP_E_STATE, "a ring label is not a known polymer id");
     if (err & 2u) return fail(ELP_E_STATE, "a ring is longer than %d", max_len);
-    for (int i = 0; i < n_pol; i++)
+    for (int i = 0; i < n_pol; i++) {
         if (lens[i] <= 0) return fail(ELP_E_STATE, "polymer id %d found %s", i, lens[i] ? "twice" : "nowhere");
+        B.n_max = lens[i] > B.n_max ? lens[i] : B.n_max;
+    }
     return ELP_OK;
 }
 
@@ -920,6 +923,34 @@ int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int 
     if (rc) return rc;
     if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
     if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return ELP_OK;
+}
+
+int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                           int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
+    if (!start3 || !sum3 || !mid3) return fail(ELP_E_ARG, "frame summary needs start3, sum3 and mid3");
+    ChainBufs B;
+    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    if (rc) return rc;
+    if (B.n_max > elp::kSummaryMaxLen)
+        return fail(ELP_E_ARG, "frame summary supports rings of up to %d monomers (found %d)", elp::kSummaryMaxLen,
+                    B.n_max);
+    HIPCHK(hipMalloc(&B.sum3.p, sizeof(int64_t) * 3 * (size_t)n_pol));
+    HIPCHK(hipMalloc(&B.mid3.p, sizeof(int32_t) * 3 * (size_t)n_pol));
+    HIPCHK(hipMalloc(&B.contacts.p, sizeof(uint64_t) * (size_t)max_len));
+    HIPCHK(hipMemsetAsync(B.contacts.p, 0, sizeof(uint64_t) * (size_t)max_len, e->stream));
+    rc = elp::chains_launch_summary((const uint8_t*)B.bonds.p, (const int32_t*)B.lens.p, n_pol, max_len, B.n_max,
+                                    (int64_t*)B.sum3.p, (int32_t*)B.mid3.p, (unsigned long long*)B.contacts.p,
+                                    e->stream);
+    if (rc) return fail(ELP_E_HIP, "summary kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(start3, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(sum3, B.sum3.p, sizeof(int64_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(mid3, B.mid3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
+    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
+    if (contacts)
+        HIPCHK(hipMemcpyAsync(contacts, B.contacts.p, sizeof(uint64_t) * max_len, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return ELP_OK;
 }

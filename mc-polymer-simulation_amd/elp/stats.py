@@ -27,6 +27,10 @@ def _load():
         lib.elp_series_rg2.argtypes = [P, I, P]
         lib.elp_series_statics.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
         lib.elp_series_msd.argtypes = [P, I, I, P, P]
+        lib.elp_series_new_summary.restype = P
+        lib.elp_series_new_summary.argtypes = [I, I, I, I, I, I, I]
+        lib.elp_series_add_summary.argtypes = [P, P, P, P, P, P, P]
+        lib.elp_series_summary.argtypes = [P, I, P, P, P, P, P, P]
         _lib = lib
     return _lib
 
@@ -34,11 +38,13 @@ def _load():
 class Series:
     """Frames of one system (n_chain chains of n_mono monomers) and their statistics."""
 
-    def __init__(self, ring, L, n_chain, n_mono, max_frames):
+    def __init__(self, ring, L, n_chain, n_mono, max_frames, summary=False):
+        """summary: keep per-chain reductions of each frame (add_summary) instead of all positions."""
         self.lib = _load()
         self.ring, self.L = bool(ring), tuple(L)
         self.n_chain, self.n_mono = int(n_chain), int(n_mono)
-        h = self.lib.elp_series_new(int(ring), *self.L, self.n_chain, self.n_mono, int(max_frames))
+        new = self.lib.elp_series_new_summary if summary else self.lib.elp_series_new
+        h = new(int(ring), *self.L, self.n_chain, self.n_mono, int(max_frames))
         if not h:
             raise ElpError("elp_series_new failed")
         self.h = C.c_void_p(h)
@@ -70,6 +76,27 @@ class Series:
             starts[c] = st
             bonds[c, :len(b)] = b[:self.n_mono]
         self.add(starts, bonds)
+
+    def add_summary(self, sm):
+        """Add one frame given as per-chain reductions (a dict as Engine.ring_frame_summary returns)."""
+        a = [np.ascontiguousarray(sm["start3"], np.int32), np.ascontiguousarray(sm["sum3"], np.int64),
+             np.ascontiguousarray(sm["mid3"], np.int32), np.ascontiguousarray(sm["rg2"], np.float64),
+             np.ascontiguousarray(sm["nsl"], np.int32), np.ascontiguousarray(sm["contacts"], np.uint64)]
+        if a[0].shape != (self.n_chain, 3) or a[5].shape[0] < self.n_mono:
+            raise ElpError("summary shape does not match the series")
+        if self.lib.elp_series_add_summary(self.h, *(x.ctypes.data for x in a)):
+            raise ElpError("elp_series_add_summary failed")
+
+    def summary(self, f):
+        """Per-chain reductions of frame f of a full series (what the device returns for that frame)."""
+        n = self.n_chain
+        out = {"start3": np.zeros((n, 3), np.int32), "sum3": np.zeros((n, 3), np.int64),
+               "mid3": np.zeros((n, 3), np.int32), "rg2": np.zeros(n), "nsl": np.zeros(n, np.int32),
+               "contacts": np.zeros(self.n_mono, np.uint64)}
+        if self.lib.elp_series_summary(self.h, int(f), *(out[k].ctypes.data for k in ("start3", "sum3", "mid3", "rg2",
+                                                                                       "nsl", "contacts"))):
+            raise ElpError("elp_series_summary failed")
+        return out
 
     def add_tuv(self, path):
         rc = self.lib.elp_series_add_tuv(self.h, os.fsencode(path))

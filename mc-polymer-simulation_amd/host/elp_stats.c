@@ -13,7 +13,14 @@ struct elp_series {
     int L[3];
     int n_chain, n_mono;
     int max_frames, n_frames;
-    int32_t* pos;        /* [frame][chain][mono][3] unwrapped t,u,v */
+    int32_t* pos;        /* [frame][chain][mono][3] unwrapped t,u,v; NULL in summary mode */
+    /* summary mode, per frame and chain (unwrapped): */
+    int32_t* first;      /* [frame][chain][3] monomer 0 */
+    int64_t* sum;        /* [frame][chain][3] sum over monomers */
+    int32_t* mid;        /* [frame][chain][3] monomer polSize/2 */
+    double* rg2s;        /* [frame][chain] */
+    int32_t* nsls;       /* [frame][chain] stored-length bonds */
+    double* cnts;        /* [frame][n_mono] contact pairs by separation */
 };
 
 static inline int32_t* frame_chain(const elp_series* s, int f, int c) {
@@ -32,9 +39,28 @@ elp_series* elp_series_new(int ring, int LT, int LU, int LV, int n_chain, int n_
     return s;
 }
 
+elp_series* elp_series_new_summary(int ring, int LT, int LU, int LV, int n_chain, int n_mono, int max_frames) {
+    if (LT <= 0 || LU <= 0 || LV <= 0 || n_chain <= 0 || n_mono < 2 || max_frames <= 0) return NULL;
+    elp_series* s = (elp_series*)calloc(1, sizeof *s);
+    if (!s) return NULL;
+    s->ring = ring ? 1 : 0;
+    s->L[0] = LT; s->L[1] = LU; s->L[2] = LV;
+    s->n_chain = n_chain; s->n_mono = n_mono; s->max_frames = max_frames;
+    const size_t fc = (size_t)max_frames * n_chain;
+    s->first = (int32_t*)malloc(sizeof(int32_t) * 3 * fc);
+    s->sum = (int64_t*)malloc(sizeof(int64_t) * 3 * fc);
+    s->mid = (int32_t*)malloc(sizeof(int32_t) * 3 * fc);
+    s->rg2s = (double*)malloc(sizeof(double) * fc);
+    s->nsls = (int32_t*)malloc(sizeof(int32_t) * fc);
+    s->cnts = (double*)malloc(sizeof(double) * (size_t)max_frames * n_mono);
+    if (!s->first || !s->sum || !s->mid || !s->rg2s || !s->nsls || !s->cnts) { elp_series_free(s); return NULL; }
+    return s;
+}
+
 void elp_series_free(elp_series* s) {
     if (!s) return;
     free(s->pos);
+    free(s->first); free(s->sum); free(s->mid); free(s->rg2s); free(s->nsls); free(s->cnts);
     free(s);
 }
 
@@ -87,9 +113,46 @@ static void place_chain(elp_series* s, int f, int c, const int32_t start[3], con
 }
 
 int elp_series_add(elp_series* s, const int32_t* starts, const uint8_t* bonds, int stride) {
-    if (!s || !starts || !bonds || stride < s->n_mono - 1) return -1;
+    if (!s || !s->pos || !starts || !bonds || stride < s->n_mono - 1) return -1;
     if (s->n_frames >= s->max_frames) return -1;
     for (int c = 0; c < s->n_chain; c++) place_chain(s, s->n_frames, c, starts + 3 * c, bonds + (size_t)c * stride);
+    s->n_frames++;
+    return 0;
+}
+
+/* Summary-mode frame: the unwrapping of place_chain done on the per-chain
+   reductions (start against the previous frame's monomer 0 by the minimum
+   image, then whole boxes so that the mean moves by less than half a box;
+   the mean is sum/n computed from the exact integer sum, as place_chain's
+   double accumulation of integer positions gives it). */
+int elp_series_add_summary(elp_series* s, const int32_t* start3, const int64_t* sum3, const int32_t* mid3,
+                           const double* rg2, const int32_t* nsl, const uint64_t* contacts) {
+    if (!s || s->pos || !start3 || !sum3 || !mid3 || !rg2 || !nsl || !contacts) return -1;
+    if (s->n_frames >= s->max_frames) return -1;
+    const int f = s->n_frames, nc = s->n_chain, n = s->n_mono;
+    for (int c = 0; c < nc; c++) {
+        const size_t at = (size_t)f * nc + c, prev = at - (size_t)nc;
+        for (int k = 0; k < 3; k++) {
+            int st = start3[3 * c + k];
+            if (f > 0) {
+                const int q0 = s->first[3 * prev + k];
+                st = q0 + imin_image(st - q0, s->L[k]);
+            }
+            int64_t S = (int64_t)n * st + sum3[3 * c + k];
+            int sh = 0;
+            if (f > 0) {
+                const double a = (double)S / n, b = (double)s->sum[3 * prev + k] / n;
+                while (a - b + sh > s->L[k] / 2) sh -= s->L[k];
+                while (a - b + sh < -s->L[k] / 2) sh += s->L[k];
+            }
+            s->first[3 * at + k] = st + sh;
+            s->sum[3 * at + k] = S + (int64_t)n * sh;
+            s->mid[3 * at + k] = st + mid3[3 * c + k] + sh;
+        }
+        s->rg2s[at] = rg2[c];
+        s->nsls[at] = nsl[c];
+    }
+    for (int g = 0; g < n; g++) s->cnts[(size_t)f * n + g] = (double)contacts[g];
     s->n_frames++;
     return 0;
 }
@@ -104,7 +167,7 @@ static int hexval(int ch) {
 /* tuv text: "LT= L\nLU= L\nLV= L\nnp= n\nmaxPolLength= M\n" then per chain
    "len= N\n t u v\t<hex x N>\n" (gpupol2/gpupol_io.c:105-203). */
 int elp_series_add_tuv(elp_series* s, const char* path) {
-    if (!s) return -1;
+    if (!s || !s->pos) return -1;
     if (s->n_frames >= s->max_frames) return -1;
     FILE* fp = fopen(path, "r");
     if (!fp) return -2;
@@ -138,7 +201,7 @@ int elp_series_add_tuv(elp_series* s, const char* path) {
 }
 
 int elp_series_positions(const elp_series* s, int f, int c, int32_t* out) {
-    if (!s || f < 0 || f >= s->n_frames || c < 0 || c >= s->n_chain || !out) return -1;
+    if (!s || !s->pos || f < 0 || f >= s->n_frames || c < 0 || c >= s->n_chain || !out) return -1;
     memcpy(out, frame_chain(s, f, c), sizeof(int32_t) * 3 * (size_t)s->n_mono);
     return 0;
 }
@@ -178,7 +241,8 @@ static double rg2_of(const elp_series* s, const int32_t* p) {
 
 int elp_series_rg2(const elp_series* s, int f, double* rg2) {
     if (!s || f < 0 || f >= s->n_frames || !rg2) return -1;
-    for (int c = 0; c < s->n_chain; c++) rg2[c] = rg2_of(s, frame_chain(s, f, c));
+    for (int c = 0; c < s->n_chain; c++)
+        rg2[c] = s->pos ? rg2_of(s, frame_chain(s, f, c)) : s->rg2s[(size_t)f * s->n_chain + c];
     return 0;
 }
 
@@ -205,7 +269,15 @@ int elp_series_statics(const elp_series* s, int n_therm, double* rg2, double* sl
     const int neq = s->n_frames - n_therm;
     double srg = 0, ssl = 0;
     double* cnt = pc ? (double*)calloc((size_t)n, sizeof(double)) : NULL;
-    for (int f = n_therm; f < s->n_frames; f++) {
+    for (int f = n_therm; f < s->n_frames && !s->pos; f++) {
+        for (int c = 0; c < s->n_chain; c++) {
+            srg += s->rg2s[(size_t)f * s->n_chain + c];
+            ssl += s->nsls[(size_t)f * s->n_chain + c] / (double)ps;
+        }
+        if (cnt)
+            for (int g = 0; g < n; g++) cnt[g] += s->cnts[(size_t)f * n + g];
+    }
+    for (int f = n_therm; f < s->n_frames && s->pos; f++) {
         for (int c = 0; c < s->n_chain; c++) {
             const int32_t* p = frame_chain(s, f, c);
             srg += rg2_of(s, p);
@@ -245,6 +317,42 @@ int elp_series_statics(const elp_series* s, int n_therm, double* rg2, double* sl
     return 0;
 }
 
+int elp_series_summary(const elp_series* s, int f, int32_t* start3, int64_t* sum3, int32_t* mid3, double* rg2,
+                       int32_t* nsl, uint64_t* contacts) {
+    if (!s || !s->pos || f < 0 || f >= s->n_frames || !start3 || !sum3 || !mid3 || !rg2 || !nsl || !contacts) return -1;
+    init_nb();
+    const int n = s->n_mono, ps = pol_size(s), mono = ps / 2;
+    for (int g = 0; g < n; g++) contacts[g] = 0;
+    for (int c = 0; c < s->n_chain; c++) {
+        const int32_t* p = frame_chain(s, f, c);
+        for (int k = 0; k < 3; k++) {
+            const int L = s->L[k];
+            start3[3 * c + k] = ((p[k] % L) + L) % L;
+            int64_t S = 0;
+            for (int i = 0; i < n; i++) S += p[3 * i + k] - p[k];
+            sum3[3 * c + k] = S;
+            mid3[3 * c + k] = p[3 * mono + k] - p[k];
+        }
+        rg2[c] = rg2_of(s, p);
+        int z = 0;
+        for (int i = 0; i < ps; i++) {
+            const int32_t* a = p + 3 * i;
+            const int32_t* b = p + 3 * ((i + 1) % n);
+            z += (a[0] == b[0] && a[1] == b[1] && a[2] == b[2]);
+        }
+        nsl[c] = z;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) {
+                const int d0 = p[3 * j] - p[3 * i], d1 = p[3 * j + 1] - p[3 * i + 1], d2 = p[3 * j + 2] - p[3 * i + 2];
+                if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1) continue;
+                int hit = (d0 == 0 && d1 == 0 && d2 == 0);
+                for (int k = 0; k < 12 && !hit; k++) hit = (d0 == g_nb[k][0] && d1 == g_nb[k][1] && d2 == g_nb[k][2]);
+                contacts[j - i] += (uint64_t)hit;
+            }
+    }
+    return 0;
+}
+
 int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3) {
     if (!s || n_therm < 0 || n_therm >= s->n_frames || n_out <= 0) return -1;
     const int nf = s->n_frames, nc = s->n_chain;
@@ -256,7 +364,16 @@ int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, doub
     for (int f = 0; f < nf; f++)
         for (int c = 0; c < nc; c++) {
             double* cc = cms + 3 * ((size_t)f * nc + c);
-            cms_of(s, frame_chain(s, f, c), cc);
+            if (s->pos) {
+                cms_of(s, frame_chain(s, f, c), cc);
+            } else {
+                /* the sum of x = t+u-v over monomers, as cms_of accumulates it */
+                const int64_t* q = s->sum + 3 * ((size_t)f * nc + c);
+                const double fac = 1.0 / (s->n_mono * sqrt(2.0));
+                cc[0] = (double)(q[0] + q[1] - q[2]) * fac;
+                cc[1] = (double)(q[0] - q[1]) * fac;
+                cc[2] = (double)q[2] * fac;
+            }
             for (int k = 0; k < 3; k++) sys[3 * f + k] += cc[k] / nc;
         }
     for (int k = 0; k < n_out; k++) {
@@ -267,8 +384,8 @@ int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, doub
             const double dr[3] = {sys[3 * t] - sys[3 * t2], sys[3 * t + 1] - sys[3 * t2 + 1], sys[3 * t + 2] - sys[3 * t2 + 2]};
             for (int c = 0; c < nc; c++) {
                 double x1, y1, z1, x2, y2, z2;
-                xyz_of(frame_chain(s, t, c) + 3 * mono, &x1, &y1, &z1);
-                xyz_of(frame_chain(s, t2, c) + 3 * mono, &x2, &y2, &z2);
+                xyz_of(s->pos ? frame_chain(s, t, c) + 3 * mono : s->mid + 3 * ((size_t)t * nc + c), &x1, &y1, &z1);
+                xyz_of(s->pos ? frame_chain(s, t2, c) + 3 * mono : s->mid + 3 * ((size_t)t2 * nc + c), &x2, &y2, &z2);
                 const double mx = (x1 - x2) * r - dr[0], my = (y1 - y2) * r - dr[1], mz = (z1 - z2) * r - dr[2];
                 a1 += mx * mx + my * my + mz * mz;
                 const double* c1 = cms + 3 * ((size_t)t * nc + c);

@@ -66,6 +66,24 @@ int elp_series_statics(const elp_series* s, int n_therm, double* rg2, double* sl
  */
 int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3);
 
+/*
+ * Summary mode: a series fed with per-chain reductions of each frame instead
+ * of all monomer positions (what elp_ring_frame_summary of libelp.so returns):
+ * per chain c its start site start3[3c..] in the box, the sum of its monomer
+ * positions relative to the start sum3[3c..], the position of monomer
+ * polSize/2 relative to the start mid3[3c..], rg2[c], the number of
+ * stored-length bonds nsl[c]; per frame contacts[g], g < n_mono (pairs g
+ * apart that touch, summed over chains).  Frames are unwrapped as in the full
+ * mode, and elp_series_statics / elp_series_msd / elp_series_rg2 give the same
+ * results (rg2 up to rounding: it is taken as given); positions are not kept.
+ */
+elp_series* elp_series_new_summary(int ring, int LT, int LU, int LV, int n_chain, int n_mono, int max_frames);
+int elp_series_add_summary(elp_series* s, const int32_t* start3, const int64_t* sum3, const int32_t* mid3,
+                           const double* rg2, const int32_t* nsl, const uint64_t* contacts);
+/* The summary of frame f of a full-mode series (start3 wrapped into the box). */
+int elp_series_summary(const elp_series* s, int f, int32_t* start3, int64_t* sum3, int32_t* mid3, double* rg2,
+                       int32_t* nsl, uint64_t* contacts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,3 +69,55 @@ def test_device_recovery_rejects_a_damaged_lattice():
     with pytest.raises(elp.ElpError, match="ring heads"):
         eng.ring_tuv(h)
     eng.close()
+
+
+@pytest.mark.parametrize("N,L,seed", [(100, 32, 7), (1000, 128, 8)])
+def test_device_frame_summary_equals_host_reduction(N, L, seed):
+    """elp_ring_frame_summary (positions, sums, middle monomer, contact histogram
+    reduced on the device) equals the host reduction of the same frame."""
+    from elp.stats import Series
+    _say(f"[frame summary] N={N} L={L}")
+    h, eng = _run(N, L, seed, 20)
+    t0 = time.perf_counter()
+    dev = eng.ring_frame_summary(h)
+    t1 = time.perf_counter()
+    h.set_lattice(eng.download())
+    full = Series(True, h.L, h.n_pol, h.N, 1)
+    full.add_host(h)
+    want = full.summary(0)
+    t2 = time.perf_counter()
+    _say(f"  device summary {1e3 * (t1 - t0):.1f} ms, download + host recovery + reduction {1e3 * (t2 - t1):.1f} ms")
+    for k in ("start3", "sum3", "mid3", "nsl", "contacts"):
+        assert np.array_equal(dev[k], want[k]), k
+    assert np.allclose(dev["rg2"], want["rg2"], rtol=1e-12, atol=0)
+    # consecutive monomers always touch: N - 1 pairs one apart per ring, and the
+    # closing pair (N - 1, 0) counts at separation N - 1
+    assert dev["contacts"][1] == h.n_pol * (N - 1) and dev["contacts"][N - 1] >= h.n_pol
+
+
+def test_device_summary_series_equals_full_series():
+    """Six frames of a melt: statics and mean-square displacements from the
+    device summaries equal those of the full host series (Rg^2 to rounding)."""
+    from elp.stats import Series
+    _say("[summary series] N=200 L=48")
+    tile = (16, 16, 16)
+    h = elp.RingHost(N=200, L=48, density=1.2, seed=9, box_unit=16)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_DENSE, 13)
+    eng.upload(h.lattice())
+    h.load_shift(tile)
+    full = Series(True, h.L, h.n_pol, h.N, 6)
+    summ = Series(True, h.L, h.n_pol, h.N, 6, summary=True)
+    tau = 0
+    for f in range(6):
+        if f:
+            eng.run(h.origins(tile, 200), tau0=tau, seed=h.seed)
+            tau += 200
+        summ.add_summary(eng.ring_frame_summary(h))
+        h.set_lattice(eng.download())
+        full.add_host(h)
+    a, b = full.statics(1), summ.statics(1)
+    assert np.isclose(a["rg2"], b["rg2"], rtol=1e-12) and a["slrat"] == b["slrat"]
+    assert np.array_equal(a["pc"], b["pc"])
+    ga, gb = full.msd(0, 6), summ.msd(0, 6)
+    assert np.array_equal(ga[0], gb[0]) and np.array_equal(ga[1], gb[1])
+    assert ga[1][5] > 0

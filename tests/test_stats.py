@@ -138,3 +138,54 @@ def test_tuv_reader_matches_recovered_chains(tmp_path):
     assert sa["rg2"] == sb["rg2"] and sa["slrat"] == sb["slrat"]
     with pytest.raises(elp.ElpError):
         Series(True, (32, 32, 32), h.n_pol + 1, h.N, 1).add_tuv(path)
+
+
+def _summary_of(full):
+    s = Series(full.ring, full.L, full.n_chain, full.n_mono, full.frames, summary=True)
+    for f in range(full.frames):
+        s.add_summary(full.summary(f))
+    return s
+
+
+def test_summary_series_rigid_translation():
+    """Per-chain reductions unwrap across the periodic boundary exactly as the positions do."""
+    L, n = 16, 8
+    full = Series(False, (L, L, L), 2, n, 5)
+    d = np.array([3, 1, 0])
+    for f in range(5):
+        st0 = (np.array([14, 2, 5]) + f * d) % L
+        full.add([st0, [1, 1, 1]], [[1] * (n - 1) + [0xF], [2] * (n - 1) + [0xF]])
+    s = _summary_of(full)
+    a, b = full.msd(0, 4), s.msd(0, 4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert full.summary(4)["start3"][0].tolist() == ((np.array([14, 2, 5]) + 4 * d) % L).tolist()
+    with pytest.raises(elp.ElpError):
+        s.summary(0)                                   # positions are not kept in summary mode
+
+
+def test_summary_series_equals_full_series_on_a_melt():
+    """Ring melt frames from the C replay (L=32, N=100): statics and mean-square
+    displacements of the summary series equal those of the full series."""
+    from ringcheck import replay_ring
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=6)
+    tile = (16, 16, 16)
+    h.load_shift(tile)
+    full = Series(True, h.L, h.n_pol, h.N, 6)
+    full.add_host(h)
+    lat, tau = h.lattice(), 0
+    for f in range(1, 6):
+        org = h.origins(tile, 25)
+        lat = replay_ring(lat, h.L, tile, elp.SUBLAT_DENSE, 13, org, tau, h.seed)
+        tau += 25
+        h.set_lattice(lat)
+        full.add_host(h)
+    s = _summary_of(full)
+    a, b = full.statics(1), s.statics(1)
+    assert a["rg2"] == b["rg2"] and a["slrat"] == b["slrat"] and np.array_equal(a["pc"], b["pc"])
+    assert a["pc"][1] > 0.5 and a["slrat"] > 0
+    ga, gb = full.msd(0, 5), s.msd(0, 5)
+    assert np.array_equal(ga[0], gb[0]) and np.array_equal(ga[1], gb[1])
+    assert ga[0][4] > ga[0][1] > 0
+    sm = full.summary(3)
+    assert np.array_equal(s.rg2(3), full.rg2(3))
+    assert sm["contacts"].sum() > 0 and sm["nsl"].sum() > 0
