@@ -65,6 +65,7 @@ int BuildSAWPolymers(SimProperties* sp, SimState* ss, int polLength, double dens
     uint* bonds = (uint*)malloc(sizeof(uint) * ((size_t)N + 1));
     uint4_t* rng = &ss->rngState;
     int rc = 0;
+    sp->maxPolLength = (uint)polLength + 1;   /* sizes each chain's arrays in AddPolymerToSim */
     long restarts = 0;
 
     for (int p = 0; p < nPol && !rc; p++) {

@@ -130,6 +130,9 @@ void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, 
     uint sl = (uint)polLength - dt * du * dv;
     if (sl >= (uint)polLength) { printf("Internal error: Failed to find polymer partition\n"); exit(1); }
     int nAdded = 0;
+    /* the final maxPolLength already here: it sizes every chain's arrays (the box
+       bound SetBoxDimension leaves, 4 LT LU, would give each chain megabytes) */
+    sp->maxPolLength = polLength + 1;
     /* note: positions step by LT/nt (not dt), as in the reference */
     for (uint t = 0; t < nt * dt && nAdded < nPol; t += sp->LT / nt)
         for (uint u = 0; u < nu * du && nAdded < nPol; u += sp->LU / nu)
@@ -213,7 +216,9 @@ void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, in
     }
     pol = ss->pol + ss->nPol;
     ss->nPol++;
+    /* room for the chain and for the head + id label bits written below */
     size_t cap = sp->maxPolLength > (uint)length + 1 ? sp->maxPolLength : (uint)length + 1;
+    if (cap < (size_t)(ss->headBit + ss->labelBit + 1)) cap = (size_t)(ss->headBit + ss->labelBit + 1);
     pol->label = (uint*)calloc(cap, sizeof(uint));
     pol->bonds = (uint*)calloc(cap, sizeof(uint));
     pol->labelStart = 0;

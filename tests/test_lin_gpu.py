@@ -83,3 +83,20 @@ def test_lin_saw_start_full_size():
     assert lin_label_sequences(got, h.L) == seq0
     assert np.count_nonzero(got != lat0) > lat0.size // 4
     h.set_lattice(got)
+
+
+def test_lin_largest_box_bit_exact():
+    """L=256 (16.7 M sites, 40 265 chains of N=500): two time units bit-exact
+    against the C replay."""
+    print("[lin parity] L=256 N=500 dense 16^3, 2 time units", flush=True)
+    h = elp.LinHost(N=500, L=256, density=1.2, seed=12)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    org = h.origins(tile, 2)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    want = replay_lin(lat0, h.L, tile, elp.SUBLAT_DENSE, 16, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > lat0.size // 4
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"

@@ -99,3 +99,22 @@ def test_ring_full_size_invariants(tile, sublat, n_iter):
     assert np.count_nonzero(got != lat0) > lat0.size // 4
     assert ring_label_sequences(got, h.L) == before
     h.set_lattice(got)
+
+
+def test_ring_largest_box_bit_exact():
+    """L=256 (16.7 M sites, 20 133 rings of N=1000, the most chains of any test):
+    two time units bit-exact against the C replay."""
+    _say("[ring parity] L=256 N=1000 dense 16^3, 2 time units")
+    h = elp.RingHost(N=1000, L=256, density=1.2, seed=11)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    h.load_shift(tile)
+    org = h.origins(tile, 2)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_DENSE, 13)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    want = replay_ring(lat0, h.L, tile, elp.SUBLAT_DENSE, 13, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > lat0.size // 4
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
+    eng.close()
