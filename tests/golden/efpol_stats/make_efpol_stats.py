@@ -1,15 +1,15 @@
 #!/usr/bin/env python3
-"""Equilibrium statistics of efpol ring melts, for the GPU statistics test.
+"""Equilibrium statistics of efpol ring or linear melts, for the GPU statistics tests.
 
-Runs the efpol restatement (oracle/bin/efpol_port_ring, byte-exact with the
-reference efpol on the golden tuv sequences) for ring melts L=32, N=100,
-density 1.2, model sl_equal, seeds 2,4,...,16 (efpol's LFSR113 seeding
-ignores bit 0 of the seed, so seeds 2k and 2k+1 give the same run), 40000
-time units with a tuv file every 1000, and writes per-seed averages over the
-last 30 frames (elp.stats: Rg^2, stored-length ratio, P(s), g1, g3) to
-ring_L32_N100.json next to this script.
+Runs the efpol restatement (oracle/bin/efpol_port_ring / efpol_port_lin,
+byte-exact with the reference efpol on the golden tuv sequences) for melts
+L=32, N=100, density 1.2, model sl_equal, seeds 2,4,...,16 (efpol's LFSR113
+seeding ignores bit 0 of the seed, so seeds 2k and 2k+1 give the same run),
+40000 time units with a tuv file every 1000, and writes per-seed averages
+over the last 30 frames (elp.stats: Rg^2, stored-length ratio, P(s), g1, g3)
+to ring_L32_N100.json or lin_L32_N100.json next to this script.
 
-    python tests/golden/efpol_stats/make_efpol_stats.py [workdir]
+    python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear] [workdir]
 """
 import glob
 import json
@@ -26,8 +26,9 @@ import numpy as np  # noqa: E402
 
 from elp.stats import Series  # noqa: E402
 
-L, N, T, INTERVAL, SEEDS = 32, 100, 40000, 1000, list(range(2, 18, 2))
-N_THERM = 11          # frames t <= 10000 are discarded
+L, N, SEEDS = 32, 100, list(range(2, 18, 2))
+T, INTERVAL = 40000, 1000             # rings; linear chains relax slower: 10x (main)
+N_THERM = 11          # the first 11 frames (t <= 10 intervals) are discarded
 LAGS = [1, 2, 4, 8, 16, 29]
 
 
@@ -45,24 +46,28 @@ def stacking(files):
 
 
 def main():
-    work = sys.argv[1] if len(sys.argv) > 1 else tempfile.mkdtemp()
-    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
+    args = sys.argv[1:]
+    kind = args.pop(0) if args and args[0] in ("ring", "linear") else "ring"
+    ring = kind == "ring"
+    work = args[0] if args else tempfile.mkdtemp()
+    t_units, interval = (T, INTERVAL) if ring else (10 * T, 10 * INTERVAL)
+    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring" if ring else "efpol_port_lin")
     procs = []
     for s in SEEDS:
         d = os.path.join(work, f"s{s}")
         os.makedirs(d, exist_ok=True)
-        procs.append(subprocess.Popen([exe, str(s), d, "1.2", str(T), str(INTERVAL), str(N), "0", str(L), "sl_equal"],
+        procs.append(subprocess.Popen([exe, str(s), d, "1.2", str(t_units), str(interval), str(N), "0", str(L), "sl_equal"],
                                       stdout=subprocess.DEVNULL))
     for p in procs:
         assert p.wait() == 0
-    out = {"L": L, "N": N, "density": 1.2, "model": "sl_equal", "time_units": T, "interval": INTERVAL,
+    out = {"L": L, "N": N, "density": 1.2, "model": "sl_equal", "time_units": t_units, "interval": interval,
            "n_therm_frames": N_THERM, "lags": LAGS, "seeds": []}
     for s in SEEDS:
         files = sorted(glob.glob(os.path.join(work, f"s{s}", f"N_{N}", "t=*_dev=0.res")),
                        key=lambda f: int(os.path.basename(f)[2:].split("_")[0]))
-        assert len(files) == T // INTERVAL + 1
+        assert len(files) == t_units // interval + 1
         npol = int(open(files[0]).read().split()[7])
-        ser = Series(True, (L, L, L), npol, N, len(files))
+        ser = Series(ring, (L, L, L), npol, N, len(files))
         for f in files:
             ser.add_tuv(f)
         st = ser.statics(N_THERM)
@@ -71,7 +76,7 @@ def main():
                              "pc": st["pc"][:N // 2 + 1].tolist(), "g1": g1[LAGS].tolist(), "g3": g3[LAGS].tolist(),
                              "stack3": stacking(files[N_THERM:])})
         print(s, st["rg2"], st["slrat"], flush=True)
-    with open(os.path.join(HERE, "ring_L32_N100.json"), "w") as f:
+    with open(os.path.join(HERE, ("ring" if ring else "lin") + "_L32_N100.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
