@@ -282,7 +282,7 @@ def main():
                     help="batch: 1 = take tile, sub-lattice, threads and units per graph from elp_autotune")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent replicas run one engine per GPU with no exchange)", file=sys.stderr)
         return 2
 

@@ -7,7 +7,7 @@
 // threads per workgroup) whose tile divides every box, on scratch copies of
 // the caller's lattices, by HIP events around the kernels.  Stage 2 times
 // the time units captured per HIP graph for the winner, by wall clock
-// (host launch cost included).  A configuration is judged by move attempts
+// (host launch cost included).  A configuration is ranked by move attempts
 // per second; the iterations per time unit follow the sub-lattice so that
 // every candidate makes about the same attempts per site per time unit.
 // The tuned schedule is returned as a struct and as a one-line text record

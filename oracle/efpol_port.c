@@ -3,8 +3,8 @@
  *
  * A single-threaded C restatement of the reference CPU ELP engine
  * (cpupol/efpol*, "efpol").  It is used (a) by the tests as the statistical
- * judge of the GPU engines and (b) by bench.py as the CPU baseline timed on
- * the host cores.  Nothing in the product links or calls this file.
+ * reference for the GPU engines and (b) by bench.py as the CPU baseline timed
+ * on the host cores.  Nothing in the product links or calls this file.
  *
  * Pinned by: byte-for-byte equality of every tuv file it writes with golden
  * files written by the reference efpol binaries (tests/golden/efpol/, made by

@@ -2,7 +2,7 @@
  * lin_replay.c -- ORACLE / TEST INFRASTRUCTURE ONLY.
  *
  * Single-threaded C replay of the linear-chain engine's schedule, the
- * bit-exact judge of mc-polymer-simulation_amd/csrc/elp_linear.hip.
+ * bit-exact reference for mc-polymer-simulation_amd/csrc/elp_linear.hip.
  *
  * Each tile is unpacked into the reference kernel's own data structures
  * (gpupol/linpol.cu:97-105, :446-476): a byte per site (next | prev<<4)
@@ -367,7 +367,7 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
  * t:0-2 u:3-5 v:6-8, :440-444), per-lane LFSR113 states rngp/rngl from and
  * back to the seed table (gpupol/gpupol_init.c:23-34); per iteration one rngl
  * draw masked by 0x88f selects the DiffuseSL site and a second one the
- * TransMove site (:483-490).  Judge: the reference kernel compiled
+ * TransMove site (:483-490).  Checked against the reference kernel compiled
  * unmodified (oracle/refkernel_lin.hip).
  */
 int orc_lin_run_ref(uint16_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {

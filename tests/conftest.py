@@ -19,7 +19,7 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build only when nothing is built (the GPU box runs on prebuilt files)."""
+    """Build only when nothing is built (a machine that receives prebuilt libraries uses them as they are)."""
     need = [os.path.join(PKG, "lib", "libelp.so"), os.path.join(PKG, "lib", "libgpupol3_ring.so"),
             os.path.join(ORACLE, "liboracle.so"), os.path.join(ORACLE, "bin", "efpol_port_ring")]
     if all(os.path.exists(p) for p in need):

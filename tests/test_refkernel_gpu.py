@@ -3,7 +3,7 @@
 oracle/_ref/refkernel_{ring,lin} run gpupol2/ringpol3.cu and gpupol/linpol.cu,
 compiled unmodified from the reference tree, with the reference launch
 sequence and seed table.  The oracle's reference-mode replays (same move
-functions as the replays that judge this build's kernels, driven by the
+functions as the replays that check the gfx950 kernels, driven by the
 reference's per-lane xorshift128 / LFSR113 states) must reproduce their
 lattice and final RNG states bit for bit.
 """
