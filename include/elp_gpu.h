@@ -208,6 +208,16 @@ int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_l
                     int32_t* lens, int32_t* starts);
 int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
                  char* text, uint64_t cap, uint64_t* len);
+/* The same for a linear-chain engine (replaces GetAllRingPolymers /
+   FindPolymerStart / GetLinPolymer / UpdatePolymerWithLabels,
+   gpupol/oclpol.c:187-287 and :700-730, and WriteLatticeFile,
+   gpupol/gpupol_io.c): a chain starts on its monomer without a prev bond;
+   per site its stored-length monomer (bond 0) comes before its carrier; the
+   last monomer's bond is 0xf. */
+int elp_lin_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds,
+                   int32_t* lens, int32_t* starts);
+int elp_lin_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
+                char* text, uint64_t cap, uint64_t* len);
 /* Per polymer id, from the same device-side walk: squared radius of gyration
    in bond units (analysis/mode_compute.c:191-217) and the number of
    stored-length bonds (ComputeSL, :145-153); either output may be NULL. */

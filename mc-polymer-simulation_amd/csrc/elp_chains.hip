@@ -149,6 +149,82 @@ __global__ void ring_text_kernel(const uint8_t* __restrict__ bonds, const int32_
     *p = '\n';
 }
 
+// ---- linear chains (uint16 site word: bits 0-3 next, 4-7 prev, 8 stored
+// length, 9 carrier label, 10 stored-length label) ----
+// Restates GetAllRingPolymers / FindPolymerStart / GetLinPolymer /
+// UpdatePolymerWithLabels of gpupol/oclpol.c (host: lin_host.c) for the
+// device: a chain starts on the site whose monomer has no prev bond (its
+// label head 1,0,0,0,0 sits on monomers 0-4); walking forward, each site
+// gives its stored-length monomer first (bond 0, label bit 10), then its
+// carrier (the site's next bond, label bit 9; 0xf at the chain end); the id
+// is the label bits of monomers 5-22, most significant first.
+
+// Starts of all chains: occupied sites without a prev bond.
+__global__ void lin_heads_kernel(const uint16_t* __restrict__ lat, uint32_t nsite, uint32_t* __restrict__ starts,
+                                 uint32_t cap, uint32_t* __restrict__ count) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nsite) return;
+    const uint32_t w = lat[x];
+    if (!(w & 0xffu) || ((w >> 4) & 0xfu)) return;
+    const uint32_t k = atomicAdd(count, 1u);
+    if (k < cap) starts[k] = x;
+}
+
+__global__ void lin_walk_kernel(const uint16_t* __restrict__ lat, Box B, const uint32_t* __restrict__ starts,
+                                uint32_t n_start, const int32_t* __restrict__ label2id, int n_pol, int max_len,
+                                uint8_t* __restrict__ bonds, int32_t* __restrict__ lens, int32_t* __restrict__ start3,
+                                int32_t* __restrict__ line_len, double* __restrict__ rg2, int32_t* __restrict__ nsl,
+                                uint32_t* __restrict__ err) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_start) return;
+    const uint32_t x0 = starts[r];
+    // pass 1: the id from the label bits of monomers 5..22
+    uint32_t x = x0, lab = 0;
+    int n = 0;
+    for (int guard = 0; n < 23 && guard < max_len; guard++) {
+        const uint32_t w = lat[x];
+        if (w & 0x100u) { if (n >= 5) lab = (lab << 1) | ((w >> 10) & 1u); n++; }
+        if (n < 23) { if (n >= 5) lab = (lab << 1) | ((w >> 9) & 1u); n++; }
+        const uint32_t nx = w & 0xfu;
+        if (!nx) break;
+        x = B.step(x, nx);
+    }
+    if (n < 23) { atomicOr(err, 4u); return; }   // shorter than head + id label
+    const int id = label2id[lab];
+    if (id < 0 || id >= n_pol) { atomicOr(err, 1u); return; }
+    // pass 2: bond string and position moments (x = t+u-v, y = t-u, z = v)
+    x = x0;
+    n = 0;
+    int pt = 0, pu = 0, pv = 0, zeros = 0;
+    double sx = 0, sy = 0, sz = 0, sq = 0;
+    uint8_t* out = bonds + (size_t)id * max_len;
+    for (;;) {
+        const uint32_t w = lat[x];
+        const double cx = pt + pu - pv, cy = pt - pu, cz = pv;
+        if (w & 0x100u) {
+            if (n >= max_len) { atomicOr(err, 2u); return; }
+            out[n++] = 0;
+            zeros++;
+            sx += cx; sy += cy; sz += cz; sq += cx * cx + cy * cy + cz * cz;
+        }
+        if (n >= max_len) { atomicOr(err, 2u); return; }
+        const uint32_t nx = w & 0xfu;
+        out[n++] = (uint8_t)(nx ? nx : 0xfu);
+        sx += cx; sy += cy; sz += cz; sq += cx * cx + cy * cy + cz * cz;
+        if (!nx) break;
+        pt += bdt(nx); pu += bdu(nx); pv += bdv(nx);
+        x = B.step(x, nx);
+    }
+    lens[id] = n;
+    rg2[id] = 0.5 * (sq / n - (sx * sx + sy * sy + sz * sz) / ((double)n * n));
+    nsl[id] = zeros;
+    const int t = (int)(x0 % (uint32_t)B.LT), u = (int)((x0 / (uint32_t)B.LT) % (uint32_t)B.LU),
+              v = (int)(x0 / ((uint32_t)B.LT * B.LU));
+    start3[3 * id] = t; start3[3 * id + 1] = u; start3[3 * id + 2] = v;
+    auto digits = [](int a) { int d = 1; while (a >= 10) { a /= 10; d++; } return d; };
+    line_len[id] = 5 + digits(n) + 1 + digits(t) + 2 + digits(u) + 2 + digits(v) + 1 + n + 1;
+}
+
 // (dt,du,dv) in {-1,0,1}^3 -> bit (dt+1) + 3(du+1) + 9(dv+1): set for the
 // same site and the 12 FCC neighbours (the displacements of the bond codes).
 constexpr uint32_t contact_mask() {
@@ -263,6 +339,22 @@ int chains_launch_walk(const uint8_t* lat, int LT, int LU, int LV, const uint32_
                        const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens, int32_t* start3,
                        int32_t* line_len, double* rg2, int32_t* nsl, uint32_t* err, hipStream_t st) {
     hipLaunchKernelGGL(ring_walk_kernel, dim3((n_start + 63) / 64), dim3(64), 0, st, lat, Box{LT, LU, LV}, starts,
+                       n_start, label2id, n_pol, max_len, bonds, lens, start3, line_len, rg2, nsl, err);
+    return (int)hipGetLastError();
+}
+
+int chains_launch_lin_heads(const uint16_t* lat, int LT, int LU, int LV, uint32_t* starts, uint32_t cap,
+                            uint32_t* count, hipStream_t st) {
+    const uint32_t n = (uint32_t)LT * LU * LV;
+    hipLaunchKernelGGL(lin_heads_kernel, dim3((n + 255) / 256), dim3(256), 0, st, lat, n, starts, cap, count);
+    return (int)hipGetLastError();
+}
+
+int chains_launch_lin_walk(const uint16_t* lat, int LT, int LU, int LV, const uint32_t* starts, uint32_t n_start,
+                           const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens,
+                           int32_t* start3, int32_t* line_len, double* rg2, int32_t* nsl, uint32_t* err,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(lin_walk_kernel, dim3((n_start + 63) / 64), dim3(64), 0, st, lat, Box{LT, LU, LV}, starts,
                        n_start, label2id, n_pol, max_len, bonds, lens, start3, line_len, rg2, nsl, err);
     return (int)hipGetLastError();
 }

@@ -857,9 +857,9 @@ struct ChainBufs {
     int n_max = 0;   // longest ring
 };
 
-static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
+static int chains_run(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
     if (!e || !label2id || n_pol <= 0 || max_len <= 0) return fail(ELP_E_ARG, "bad chain-recovery arguments");
-    if (e->kind != 0) return fail(ELP_E_ARG, "chain recovery needs a ring engine");
+    if (e->kind != kind) return fail(ELP_E_ARG, "chain recovery needs a %s engine", kind ? "linear" : "ring");
     if (!e->uploaded) return fail(ELP_E_STATE, "chain recovery before upload");
     HIPCHK(hipSetDevice(e->device));
     const size_t nl = (size_t)1 << 18;
@@ -878,25 +878,35 @@ static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max
     std::vector<int32_t> neg(n_pol, 0);
     HIPCHK(hipMemcpyAsync(B.lens.p, neg.data(), sizeof(int32_t) * n_pol, hipMemcpyHostToDevice, e->stream));
     uint32_t* cnt = (uint32_t*)B.count.p;
-    int rc = elp::chains_launch_heads((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
-                                      (uint32_t)n_pol, cnt, e->stream);
+    int rc = kind ? elp::chains_launch_lin_heads((const uint16_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
+                                                 (uint32_t)n_pol, cnt, e->stream)
+                  : elp::chains_launch_heads((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
+                                             (uint32_t)n_pol, cnt, e->stream);
     if (rc) return fail(ELP_E_HIP, "head kernel: %s", hipGetErrorString((hipError_t)rc));
     uint32_t nfound = 0;
     HIPCHK(hipMemcpyAsync(&nfound, cnt, sizeof nfound, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (nfound != (uint32_t)n_pol) return fail(ELP_E_STATE, "found %u ring heads, expected %d", nfound, n_pol);
-    rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p, nfound,
-                                 (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p, (int32_t*)B.lens.p,
-                                 (int32_t*)B.start3.p, (int32_t*)B.line_len.p, (double*)B.rg2.p, (int32_t*)B.nsl.p,
-                                 cnt + 1, e->stream);
+    if (nfound != (uint32_t)n_pol)
+        return fail(ELP_E_STATE, "found %u %s, expected %d", nfound, kind ? "chain starts" : "ring heads", n_pol);
+    if (kind)
+        rc = elp::chains_launch_lin_walk((const uint16_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p,
+                                         nfound, (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p,
+                                         (int32_t*)B.lens.p, (int32_t*)B.start3.p, (int32_t*)B.line_len.p,
+                                         (double*)B.rg2.p, (int32_t*)B.nsl.p, cnt + 1, e->stream);
+    else
+        rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p, nfound,
+                                     (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p, (int32_t*)B.lens.p,
+                                     (int32_t*)B.start3.p, (int32_t*)B.line_len.p, (double*)B.rg2.p, (int32_t*)B.nsl.p,
+                                     cnt + 1, e->stream);
     if (rc) return fail(ELP_E_HIP, "walk kernel: %s", hipGetErrorString((hipError_t)rc));
     uint32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, cnt + 1, sizeof err, hipMemcpyDeviceToHost, e->stream));
     std::vector<int32_t> lens(n_pol);
     HIPCHK(hipMemcpyAsync(lens.data(), B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (err & 1u) return fail(ELP_E_STATE, "a ring label is not a known polymer id");
-    if (err & 2u) return fail(ELP_E_STATE, "a ring is longer than %d", max_len);
+    if (err & 1u) return fail(ELP_E_STATE, "a chain label is not a known polymer id");
+    if (err & 2u) return fail(ELP_E_STATE, "a chain is longer than %d", max_len);
+    if (err & 4u) return fail(ELP_E_STATE, "a chain is shorter than its 23 head and id labels");
     for (int i = 0; i < n_pol; i++) {
         if (lens[i] <= 0) return fail(ELP_E_STATE, "polymer id %d found %s", i, lens[i] ? "twice" : "nowhere");
         B.n_max = lens[i] > B.n_max ? lens[i] : B.n_max;
@@ -904,10 +914,10 @@ static int chains_run(elp_engine* e, const int32_t* label2id, int n_pol, int max
     return ELP_OK;
 }
 
-int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens,
-                    int32_t* starts) {
+static int chains_get(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds,
+                      int32_t* lens, int32_t* starts) {
     ChainBufs B;
-    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    int rc = chains_run(e, kind, label2id, n_pol, max_len, B);
     if (rc) return rc;
     if (bonds) HIPCHK(hipMemcpyAsync(bonds, B.bonds.p, (size_t)n_pol * max_len, hipMemcpyDeviceToHost, e->stream));
     if (lens) HIPCHK(hipMemcpyAsync(lens, B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
@@ -916,10 +926,20 @@ int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_l
     return ELP_OK;
 }
 
+int elp_ring_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens,
+                    int32_t* starts) {
+    return chains_get(e, 0, label2id, n_pol, max_len, bonds, lens, starts);
+}
+
+int elp_lin_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds, int32_t* lens,
+                   int32_t* starts) {
+    return chains_get(e, 1, label2id, n_pol, max_len, bonds, lens, starts);
+}
+
 int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, double* rg2,
                          int32_t* n_sl) {
     ChainBufs B;
-    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    int rc = chains_run(e, 0, label2id, n_pol, max_len, B);
     if (rc) return rc;
     if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
     if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
@@ -931,7 +951,7 @@ int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, in
                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
     if (!start3 || !sum3 || !mid3) return fail(ELP_E_ARG, "frame summary needs start3, sum3 and mid3");
     ChainBufs B;
-    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    int rc = chains_run(e, 0, label2id, n_pol, max_len, B);
     if (rc) return rc;
     if (B.n_max > elp::kSummaryMaxLen)
         return fail(ELP_E_ARG, "frame summary supports rings of up to %d monomers (found %d)", elp::kSummaryMaxLen,
@@ -955,11 +975,11 @@ int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, in
     return ELP_OK;
 }
 
-int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
-                 char* text, uint64_t cap, uint64_t* len) {
+static int tuv_text(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len,
+                    int header_max_pol_length, char* text, uint64_t cap, uint64_t* len) {
     if (!len) return fail(ELP_E_ARG, "len is NULL");
     ChainBufs B;
-    int rc = chains_run(e, label2id, n_pol, max_len, B);
+    int rc = chains_run(e, kind, label2id, n_pol, max_len, B);
     if (rc) return rc;
     std::vector<int32_t> ll(n_pol);
     HIPCHK(hipMemcpyAsync(ll.data(), B.line_len.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
@@ -984,6 +1004,16 @@ int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len,
     HIPCHK(hipMemcpyAsync(text + hn, B.text.p, total, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return ELP_OK;
+}
+
+int elp_ring_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
+                 char* text, uint64_t cap, uint64_t* len) {
+    return tuv_text(e, 0, label2id, n_pol, max_len, header_max_pol_length, text, cap, len);
+}
+
+int elp_lin_tuv(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int header_max_pol_length,
+                char* text, uint64_t cap, uint64_t* len) {
+    return tuv_text(e, 1, label2id, n_pol, max_len, header_max_pol_length, text, cap, len);
 }
 
 int elp_net_patch_moves(int device, const uint8_t* patches, const int32_t* moves, int count, uint8_t* out,

@@ -103,6 +103,8 @@ def load_elp():
         lib.elp_ring_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
         lib.elp_ring_chain_stats.argtypes = [P, P, I, I, P, P]
         lib.elp_ring_frame_summary.argtypes = [P, P, I, I, P, P, P, P, P, P]
+        lib.elp_lin_chains.argtypes = [P, P, I, I, P, P, P]
+        lib.elp_lin_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
         lib.elp_net_create.argtypes = [C.POINTER(P), I, U32]
         lib.elp_net_destroy.argtypes = [P]
         lib.elp_net_params.argtypes = [P, P, U64]
@@ -250,13 +252,30 @@ class Engine:
 
     def ring_tuv(self, host):
         """The tuv file text of the device lattice (bytes), from the device-side recovery."""
+        return self._tuv(self.lib.elp_ring_tuv, host)
+
+    def lin_chains(self, host):
+        """Linear chains of the device lattice, recovered on the device: (bonds[n_pol][N], lengths,
+        starts[n_pol][3]) in polymer-id order; the last bond of a chain is 0xf."""
+        l2i = host.label2id()
+        bonds = np.zeros((host.n_pol, host.N), np.uint8)
+        lens = np.zeros(host.n_pol, np.int32)
+        starts = np.zeros((host.n_pol, 3), np.int32)
+        _check(self.lib.elp_lin_chains(self.h, l2i.ctypes.data, host.n_pol, host.N, bonds.ctypes.data,
+                                       lens.ctypes.data, starts.ctypes.data), self.lib)
+        return bonds, lens, starts
+
+    def lin_tuv(self, host):
+        """The tuv file text of a linear-chain device lattice (bytes), from the device-side recovery."""
+        return self._tuv(self.lib.elp_lin_tuv, host)
+
+    def _tuv(self, fn, host):
         l2i = host.label2id()
         n = C.c_uint64()
-        _check(self.lib.elp_ring_tuv(self.h, l2i.ctypes.data, host.n_pol, host.N, host.max_pol_length, None, 0,
-                                     C.byref(n)), self.lib)
+        _check(fn(self.h, l2i.ctypes.data, host.n_pol, host.N, host.max_pol_length, None, 0, C.byref(n)), self.lib)
         buf = C.create_string_buffer(n.value)
-        _check(self.lib.elp_ring_tuv(self.h, l2i.ctypes.data, host.n_pol, host.N, host.max_pol_length, buf, n.value,
-                                     C.byref(n)), self.lib)
+        _check(fn(self.h, l2i.ctypes.data, host.n_pol, host.N, host.max_pol_length, buf, n.value, C.byref(n)),
+               self.lib)
         return buf.raw[:n.value]
 
 
@@ -537,20 +556,20 @@ class _Host:
     def seed(self):
         return getattr(self.lib, self.prefix + "seed")(self.h)
 
-
-class RingHost(_Host):
-    prefix = "elp_ring_host_"
-    libname = "libgpupol3_ring.so"
-    dtype = np.uint8
-
     def label2id(self):
-        """the 2^18-entry table from a ring's 18 id label bits to its polymer id (-1 unused)"""
-        f = self.lib.elp_ring_host_label2id
+        """the 2^18-entry table from a chain's 18 id label bits to its polymer id (-1 unused)"""
+        f = getattr(self.lib, self.prefix + "label2id")
         f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
         out = np.empty(1 << 18, np.int32)
         if f(self.h, out.ctypes.data, out.size):
             raise ElpError("label table not available")
         return out
+
+
+class RingHost(_Host):
+    prefix = "elp_ring_host_"
+    libname = "libgpupol3_ring.so"
+    dtype = np.uint8
 
 
 class LinHost(_Host):

@@ -79,7 +79,7 @@ int GPULatticeToCPU(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, 
 }
 
 /* gpupol/gpupol_cuda.cu:84-112 */
-int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime) {
+static void run_units(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, int nTime) {
     int32_t* org = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(nTime > 0 ? nTime : 1));
     for (int d = 0; d < sp->nDevices; d++) {
         GPUDeviceState* dev = devStates + d;
@@ -97,7 +97,40 @@ int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULib
         }
     }
     free(org);
+}
+
+int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime) {
+    run_units(sp, ss, devStates, nTime);
     GPULatticeToCPU(sp, ss, devStates, ctx);
+    return 0;
+}
+
+/* As GPULibRun without the copy back and host recovery: the caller writes
+   the tuv file from the device (WriteLatticeFileDevice). */
+int GPULibRunDevice(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime) {
+    (void)ctx;
+    run_units(sp, ss, devStates, nTime);
+    return 0;
+}
+
+/* Device-side chain recovery and tuv text unless ELP_GPU_TUV=0. */
+int ElpGpuTuvEnabled(void) {
+    const char* env = getenv("ELP_GPU_TUV");
+    return !(env && env[0] == '0');
+}
+
+int WriteLatticeFileDevice(SimProperties* sp, SimState* ss, GPUDeviceState* dev, const char* file) {
+    elp_engine* e = (elp_engine*)dev->engine;
+    uint64_t len = 0;
+    const int maxLen = sp->polLength > 0 ? sp->polLength : (int)sp->maxPolLength;
+    ELPCHK(elp_lin_tuv(e, ss->label2Id, ss->nPol, maxLen, (int)sp->maxPolLength, NULL, 0, &len));
+    char* text = (char*)malloc(len ? len : 1);
+    ELPCHK(elp_lin_tuv(e, ss->label2Id, ss->nPol, maxLen, (int)sp->maxPolLength, text, len, &len));
+    FILE* f = fopen(file, "w");
+    if (!f) { printf("Error opening file %s\n", file); free(text); return -1; }
+    fwrite(text, 1, len, f);
+    fclose(f);
+    free(text);
     return 0;
 }
 
@@ -195,6 +228,12 @@ int elp_lin_host_polymer(const elp_lin_host* h, int id, uint32_t* bonds, int cap
         return pol->length;
     }
     return -1;
+}
+
+int elp_lin_host_label2id(const elp_lin_host* h, int32_t* out, uint64_t n) {
+    if (n != ((uint64_t)1 << h->ss.labelBit) || !h->ss.label2Id) return -1;
+    for (uint64_t i = 0; i < n; i++) out[i] = h->ss.label2Id[i];
+    return 0;
 }
 
 /* the linear driver draws no load-time shift */

@@ -111,6 +111,11 @@ int GPULibInit(SimProperties* sp, GPUDeviceState* devStates, GPULibContext* ctx)
 int GPULibLoadBuffers(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx);
 int GPULibRelease(SimProperties* sp, GPUDeviceState* devStates, GPULibContext* ctx);
 int GPULibRun(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime);
+/* GPULibRun without the copy back; the tuv file is then written from the
+   device (device-side chain recovery, unless ELP_GPU_TUV=0). */
+int GPULibRunDevice(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx, int nTime);
+int ElpGpuTuvEnabled(void);
+int WriteLatticeFileDevice(SimProperties* sp, SimState* ss, GPUDeviceState* dev, const char* file);
 int GPULatticeToCPU(SimProperties* sp, SimState* ss, GPUDeviceState* devStates, GPULibContext* ctx);
 void ElpLinScheduleFromEnv(SimProperties* sp, GPUDeviceState* dev);
 #ifdef __cplusplus

@@ -5,6 +5,8 @@
  * As in the reference, FAST_EQ reshuffles the stored length of the host copy
  * before each interval (gpupol/gpupol.c:54); the device lattice is not
  * reloaded, so it only advances the host random stream.
+ * tuv files are written from the device (chain recovery and text on the
+ * GPU, byte-identical) unless ELP_GPU_TUV=0 or FAST_EQ is set.
  * ELP_LIN_INIT=saw[:FRACTION] makes a fresh start grow the chains as random
  * self-avoiding walks covering FRACTION of the sites (default
  * ELP_SAW_SITE_FRACTION) instead of the reference's folded chains; resumed
@@ -51,12 +53,15 @@ int main(int argc, char** argv) {
     GPULibLoadBuffers(&sp, ss, devices, &gpuContext);
     const char* clean = "\b \b\b \b\b \b\b \r";
     double totStep = (double)(sp.tMax - sp.curT);
+    /* FAST_EQ reshuffles the host copy each interval, which must then be current */
+    const int deviceTuv = ElpGpuTuvEnabled() && !sp.fastEq;
     double kernelMs = 0;
     long units = 0;
     for (; sp.curT < sp.tMax; sp.curT += sp.writeInterval) {
         gettimeofday(&t0, NULL);
         if (sp.fastEq) RedistribSL(ss, &sp);
-        GPULibRun(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
+        if (deviceTuv) GPULibRunDevice(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
+        else GPULibRun(&sp, ss, devices, &gpuContext, (int)sp.writeInterval);
         gettimeofday(&t1, NULL);
         kernelMs += devices[0].lastKernelMs;
         units += sp.writeInterval;
@@ -67,7 +72,8 @@ int main(int argc, char** argv) {
         fflush(stdout);
         for (int i = 0; i < sp.nDevices; i++) {
             snprintf(fileOut, sizeof fileOut, "%s/t=%li_dev=%i.res", sp.dirOut, sp.curT + sp.writeInterval, i);
-            WriteLatticeFile(&sp, ss + i, fileOut);
+            if (deviceTuv) WriteLatticeFileDevice(&sp, ss + i, devices + i, fileOut);
+            else WriteLatticeFile(&sp, ss + i, fileOut);
         }
     }
     printf("\nSimulation complete\n");

@@ -121,3 +121,47 @@ def test_device_summary_series_equals_full_series():
     ga, gb = full.msd(0, 6), summ.msd(0, 6)
     assert np.array_equal(ga[0], gb[0]) and np.array_equal(ga[1], gb[1])
     assert ga[1][5] > 0
+
+
+def _run_lin(N, L, seed, n_time, init="reference"):
+    tile = (16, 16, 16)
+    h = elp.LinHost(N=N, L=L, density=1.2, seed=seed, init=init)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(h.lattice())
+    eng.run(h.origins(tile, n_time), tau0=0, seed=h.seed)
+    return h, eng
+
+
+@pytest.mark.parametrize("N,L,seed,n_time,init", [(100, 32, 3, 50, "reference"), (200, 64, 4, 20, "reference"),
+                                                  (500, 128, 5, 30, "reference"), (500, 128, 6, 30, "saw")])
+def test_device_lin_tuv_equals_host_writer(tmp_path, N, L, seed, n_time, init):
+    """Linear chains: device-side recovery and text equal the host recovery and writer byte for byte."""
+    _say(f"[linear chains] N={N} L={L} start={init}")
+    h, eng = _run_lin(N, L, seed, n_time, init)
+    t0 = time.perf_counter()
+    text = eng.lin_tuv(h)
+    t1 = time.perf_counter()
+    h.set_lattice(eng.download())
+    path = tmp_path / "t.res"
+    h.write_tuv(str(path))
+    t2 = time.perf_counter()
+    want = path.read_bytes()
+    _say(f"  device recovery + text {1e3 * (t1 - t0):.1f} ms, host download + recovery + writer "
+         f"{1e3 * (t2 - t1):.1f} ms, {len(want)} bytes")
+    assert text == want
+    bonds, lens, starts = eng.lin_chains(h)
+    for pid in range(0, h.n_pol, max(1, h.n_pol // 50)):
+        st, b = h.polymer(pid)
+        assert lens[pid] == len(b) and tuple(starts[pid]) == st and np.array_equal(bonds[pid, :len(b)], b)
+    eng.close()
+
+
+def test_device_lin_recovery_rejects_a_damaged_lattice():
+    h, eng = _run_lin(100, 32, 6, 5)
+    lat = eng.download()
+    occ = (lat & 0xFF) != 0
+    lat[occ] ^= 0x200                           # every carrier label flipped: no known chain id left
+    eng.upload(lat)
+    with pytest.raises(elp.ElpError, match="known polymer id"):
+        eng.lin_tuv(h)
+    eng.close()

@@ -84,6 +84,11 @@ def test_linear_dropin(tmp_path):
     assert len(chains) == 197 and all(c.endswith("f") and len(c) == 200 for c in chains)
     r = elp.LinHost(path=str(d / "t=200_dev=0.res"))
     assert r.n_pol == 197
+    # the files come from the device-side recovery; the host path (ELP_GPU_TUV=0) writes the same bytes
+    d2 = tmp_path / "lin2"
+    _run("gpupol_cuda_lin", [200, 200, 5, d2, 1.2, 0, 100, 32, 32, 32], env=dict(os.environ, ELP_GPU_TUV="0"))
+    for t in (0, 100, 200):
+        assert filecmp.cmp(d / f"t={t}_dev=0.res", d2 / f"t={t}_dev=0.res", shallow=False), t
 
 
 def test_linear_dropin_saw_start(tmp_path):
