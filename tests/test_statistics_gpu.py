@@ -35,6 +35,8 @@ L, N, T, INTERVAL, N_THERM, LAGS = REF["L"], REF["N"], REF["time_units"], REF["i
 S_POINTS = [2, 3, 5, 10, 20, 50]
 SCHEDULES = {"reference 32^3 cubic 51": ((32, 32, 32), elp.SUBLAT_CUBIC, 51),
              "dense 16^3 13": ((16, 16, 16), elp.SUBLAT_DENSE, 13)}
+# the reference's cubic sub-lattice on smaller tiles (8x the workgroups of 32^3)
+OTHER = {"cubic 16^3 51": ((16, 16, 16), elp.SUBLAT_CUBIC, 51)}
 _cache = {}
 
 
@@ -46,7 +48,7 @@ def _say(*a):
 def _run(name, rules=elp.RULES_REFERENCE):
     if (name, rules) in _cache:
         return _cache[(name, rules)]
-    tile, sub, n_iter = SCHEDULES[name]
+    tile, sub, n_iter = {**SCHEDULES, **OTHER}[name]
     res = []
     for seed in range(8):
         _say(f"[statistics] {name} rules {rules} seed {seed}")
@@ -112,6 +114,19 @@ def test_reference_rules_schedule_dependence():
     zs = _compare_schedules(elp.RULES_REFERENCE)
     assert abs(zs["rg2"]) < 3, zs
     assert zs["slrat"] < -3, zs
+
+
+def test_reference_rules_cubic_tile_size_independent():
+    """Reference rates: the cubic sub-lattice on 16^3 tiles samples the same
+    statics as on the reference's 32^3 tiles (only the sub-lattice matters)."""
+    a, b = _run("reference 32^3 cubic 51"), _run("cubic 16^3 51")
+    for key in ("rg2", "slrat"):
+        z = _z([r[key] for r in a], [r[key] for r in b])
+        _say(f"cubic 32^3 vs 16^3 {key}: z = {z:.2f}")
+        assert abs(z) < 3, (key, z)
+    z = _z([r["pc"][S_POINTS] for r in a], [r["pc"][S_POINTS] for r in b])
+    _say(f"cubic 32^3 vs 16^3 P(s): z = {np.round(z, 2)}")
+    assert np.all(np.abs(z) < 3), z
 
 
 @pytest.mark.parametrize("rules", [elp.RULES_REFERENCE, elp.RULES_BALANCED])
