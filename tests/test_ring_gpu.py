@@ -118,3 +118,41 @@ def test_ring_largest_box_bit_exact():
     assert np.count_nonzero(want != lat0) > lat0.size // 4
     assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
     eng.close()
+
+
+@pytest.mark.parametrize("kind", ["ring", "linear"])
+def test_empty_lattice_and_zero_time_units(kind):
+    """An empty box stays empty; a run of zero time units changes nothing."""
+    _say(f"[edge cases] {kind}: empty lattice, zero time units")
+    dtype = np.uint8 if kind == "ring" else np.uint16
+    tile = (16, 16, 16)
+    eng = elp.Engine(kind, (32, 32, 32), tile, elp.SUBLAT_DENSE, 13 if kind == "ring" else 16)
+    eng.upload(np.zeros(32 ** 3, dtype))
+    org = np.random.default_rng(1).integers(0, 16, size=(5, 3)).astype(np.int32)
+    eng.run(org, tau0=0, seed=3)
+    assert not eng.download().any()
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=2) if kind == "ring" else elp.LinHost(N=100, L=32, density=1.2,
+                                                                                             seed=2)
+    lat0 = h.lattice()
+    eng.upload(lat0)
+    eng.run(np.zeros((0, 3), np.int32), tau0=0, seed=3)
+    assert np.array_equal(eng.download(), lat0)
+    eng.close()
+
+
+def test_ring_non_cubic_box_bit_exact():
+    """A 64 x 32 x 48 box (edges of different tile multiples) on the solo engine."""
+    _say("[ring parity] box 64x32x48 dense 16^3")
+    h = elp.RingHost(N=200, L=(64, 32, 48), density=1.2, seed=5, box_unit=16)
+    assert h.L == (64, 32, 48)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    h.load_shift(tile)
+    org = h.origins(tile, 8)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_DENSE, 13)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    want = replay_ring(lat0, h.L, tile, elp.SUBLAT_DENSE, 13, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > 1000
+    assert np.array_equal(eng.download(), want)
+    eng.close()
