@@ -12,14 +12,19 @@ Prints one JSON line.  value = move attempts/s (every slot of every substep
 counts as one attempt, as in the reference kernel).  roofline.achieved =
 algorithmic HBM bytes per launch (the tile is read and written once:
 2 B/site x L^3) / average launch time from HIP events on the engine's stream.
-cpu_baseline = the efpol restatement (oracle/bin/efpol_port_ring) timed on
-the host cores, one independent process per core, on a bounded sample.
+cpu_baseline = the reference CPU engine efpol (cpupol/, built from its own
+sources into oracle/_ref) timed on the host cores, one independent process
+per core, on a bounded sample; the byte-exact efpol restatement
+(oracle/bin/efpol_port_ring) when that build is absent.
 """
 import argparse
 import json
 import os
+import re
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -31,7 +36,44 @@ sys.path.insert(0, PKG)
 HBM_PEAK_GBS = 8000.0
 
 
+def cpu_baseline_reference(L, N, density, cores, warm, timed, ring=True):
+    """The reference efpol itself (cpupol/, built from its sources into oracle/_ref by
+    oracle/Makefile.ref): `cores` independent runs of warm + timed time units with a write
+    interval of `timed`; each prints its DoMCStep rate per interval (cpupol/efpol.c:92-95),
+    and the rate of its last interval (the timed one) is summed."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "efpol_ring" if ring else "efpol_lin")
+    if not os.path.exists(exe) or warm % timed:
+        return None
+    work = tempfile.mkdtemp(prefix="efpol_ref_")
+    try:
+        procs = [subprocess.Popen([exe, str(2000 + 2 * i), os.path.join(work, str(i)), str(density), str(warm + timed),
+                                   str(timed), str(N), "0", str(L), "sl_equal"],
+                                  stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for i in range(cores)]
+        rate = 0.0
+        for p in procs:
+            out, _ = p.communicate()
+            rates = re.findall(r"at ([0-9.]+) M/s", out)
+            if p.returncode != 0 or len(rates) != (warm + timed) // timed:
+                return None
+            rate += float(rates[-1]) * 1e6
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return {
+        "value": rate,
+        "unit": "attempts/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": (f"reference efpol (cpupol, built from its sources), {'ring' if ring else 'linear'} L={L} N={N} "
+                   f"density {density}, {cores} independent processes (seeds 2000, 2002, ...), {warm} untimed + "
+                   f"{timed} timed time units from efpol's own start; its DoMCStep rate of the timed interval; "
+                   f"1 attempt = 1 monomer move attempt"),
+    }
+
+
 def cpu_baseline(L, N, density, cores, warm, timed, ring=True):
+    ref = cpu_baseline_reference(L, N, density, cores, warm, timed, ring)
+    if ref:
+        return ref
     exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring" if ring else "efpol_port_lin")
     if not os.path.exists(exe):
         return None
