@@ -107,6 +107,8 @@ static int create(elp_engine** out, int kind, int device, int LT, int LU, int LV
         return fail(ELP_E_ARG, "tile %dx%dx%d: edges must be multiples of 4 in [8,32]", tt, tu, tv);
     if (LT <= 0 || LU <= 0 || LV <= 0 || LT % tt || LU % tu || LV % tv)
         return fail(ELP_E_ARG, "box %dx%dx%d not divisible by tile %dx%dx%d", LT, LU, LV, tt, tu, tv);
+    if ((uint64_t)LT * LU * LV >= ((uint64_t)1 << 31))
+        return fail(ELP_E_ARG, "box %dx%dx%d: at most 2^31 - 1 sites (the kernels index sites in 32 bits)", LT, LU, LV);
     if (n_iter < 0) return fail(ELP_E_ARG, "n_iter < 0");
     elp_engine* e = new elp_engine();
     e->kind = kind; e->device = device;
@@ -326,6 +328,10 @@ static int batch_create(elp_batch** out, int kind, int device, int n_rep, const 
         if (LT <= 0 || LU <= 0 || LV <= 0 || LT % tt || LU % tu || LV % tv) {
             delete b;
             return fail(ELP_E_ARG, "replica %d: box %dx%dx%d not divisible by tile %dx%dx%d", r, LT, LU, LV, tt, tu, tv);
+        }
+        if ((uint64_t)LT * LU * LV >= ((uint64_t)1 << 31)) {
+            delete b;
+            return fail(ELP_E_ARG, "replica %d: box %dx%dx%d has more than 2^31 - 1 sites", r, LT, LU, LV);
         }
         elp::BatchRep rep{};
         rep.site_off = site;

@@ -36,6 +36,48 @@ __device__ __forceinline__ UnitArgs unit_args(const RingLaunch& p) {
                     p.rules};
 }
 
+// Copy one tile between the box (row-major, periodic) and LDS, in either
+// direction.  Site indices are 32-bit (boxes hold fewer than 2^31 sites).
+// When the workgroup covers whole (t,u) planes (THREADS a multiple of
+// TT*TU), each thread keeps its (t,u) and steps v: the box index advances by
+// one plane per step and wraps once, one add and one conditional subtract.
+template <bool LOAD, int TT, int TU, int TV, int THREADS, int PAD, class T>
+__device__ __forceinline__ void tile_copy(T* __restrict__ glat, T* tile, const UnitArgs& p, int gt0, int gu0, int gv0,
+                                          int tid) {
+    using G = PTile<TT, TU, TV, PAD>;
+    const uint32_t plane = (uint32_t)p.LT * (uint32_t)p.LU, nsite = plane * (uint32_t)p.LV;
+    if constexpr (THREADS % (TT * TU) == 0) {
+        constexpr int VSTEP = THREADS / (TT * TU);
+        const uint32_t lt = tid & (TT - 1), lu = (tid >> G::LT) & (TU - 1), lv0 = (uint32_t)tid >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv0;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        uint32_t idx = (uint32_t)gt + (uint32_t)p.LT * (uint32_t)gu + plane * (uint32_t)gv;
+#pragma unroll
+        for (int k = 0; k < TV / VSTEP; k++) {
+            const uint32_t lv = lv0 + (uint32_t)(k * VSTEP);
+            const uint32_t a = PAD ? G::pack(lt, lu, lv) : lt + TT * (lu + TU * lv);
+            if (LOAD) tile[a] = glat[idx];
+            else glat[idx] = tile[a];
+            idx += plane * VSTEP;
+            idx -= (idx >= nsite) ? nsite : 0u;
+        }
+    } else {
+        for (int i = tid; i < TT * TU * TV; i += THREADS) {
+            const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
+            int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
+            gt -= (gt >= p.LT) ? p.LT : 0;
+            gu -= (gu >= p.LU) ? p.LU : 0;
+            gv -= (gv >= p.LV) ? p.LV : 0;
+            const uint32_t idx = (uint32_t)gt + (uint32_t)p.LT * ((uint32_t)gu + (uint32_t)p.LU * (uint32_t)gv);
+            const uint32_t a = PAD ? G::pack(lt, lu, lv) : (uint32_t)i;
+            if (LOAD) tile[a] = glat[idx];
+            else glat[idx] = tile[a];
+        }
+    }
+}
+
 template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
 __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat, typename M::site_t* tile,
                                           const UnitArgs& p, uint32_t tileId) {
@@ -55,15 +97,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
 
     // HBM -> LDS (periodic wrap; origins are < one tile edge, so one subtraction suffices)
-    for (int i = tid; i < TT * TU * TV; i += THREADS) {
-        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
-        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
-        gt -= (gt >= p.LT) ? p.LT : 0;
-        gu -= (gu >= p.LU) ? p.LU : 0;
-        gv -= (gv >= p.LV) ? p.LV : 0;
-        tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i] =
-            glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
-    }
+    tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
 
     // slot k of this thread: cell corner (+ j along t for the dense cosets)
     uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
@@ -106,15 +140,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     }
 
     // LDS -> HBM
-    for (int i = tid; i < TT * TU * TV; i += THREADS) {
-        const uint32_t lt = i & (TT - 1), lu = (i >> G::LT) & (TU - 1), lv = i >> (G::LT + G::LU);
-        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv;
-        gt -= (gt >= p.LT) ? p.LT : 0;
-        gu -= (gu >= p.LU) ? p.LU : 0;
-        gv -= (gv >= p.LV) ? p.LV : 0;
-        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] =
-            tile[PAD ? G::pack(lt, lu, lv) : (uint32_t)i];
-    }
+    tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
 }
 
 // Workgroup of a batched launch: global tile g (XCD-aware order over all
