@@ -41,6 +41,9 @@ __device__ __forceinline__ UnitArgs unit_args(const RingLaunch& p) {
 // When the workgroup covers whole (t,u) planes (THREADS a multiple of
 // TT*TU), each thread keeps its (t,u) and steps v: the box index advances by
 // one plane per step and wraps once, one add and one conditional subtract.
+// When it covers whole t-rows of a plane (THREADS a multiple of TT dividing
+// TT*TU), each thread keeps its t, holds the row offsets of its NU rows of a
+// plane, and steps v the same way.
 template <bool LOAD, int TT, int TU, int TV, int THREADS, int PAD, class T>
 __device__ __forceinline__ void tile_copy(T* __restrict__ glat, T* tile, const UnitArgs& p, int gt0, int gu0, int gv0,
                                           int tid) {
@@ -62,6 +65,33 @@ __device__ __forceinline__ void tile_copy(T* __restrict__ glat, T* tile, const U
             else glat[idx] = tile[a];
             idx += plane * VSTEP;
             idx -= (idx >= nsite) ? nsite : 0u;
+        }
+    } else if constexpr (THREADS % TT == 0 && (TT * TU) % THREADS == 0) {
+        constexpr int USTEP = THREADS / TT, NU = TU / USTEP;
+        const uint32_t lt = tid & (TT - 1), lu0 = (uint32_t)tid >> G::LT;
+        int gt = gt0 + (int)lt;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        uint32_t row[NU];
+#pragma unroll
+        for (int j = 0; j < NU; j++) {
+            int gu = gu0 + (int)lu0 + j * USTEP;
+            gu -= (gu >= p.LU) ? p.LU : 0;
+            row[j] = (uint32_t)gt + (uint32_t)p.LT * (uint32_t)gu;
+        }
+        int gv = gv0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        uint32_t base = plane * (uint32_t)gv;
+#pragma unroll
+        for (int lv = 0; lv < TV; lv++) {
+#pragma unroll
+            for (int j = 0; j < NU; j++) {
+                const uint32_t lu = lu0 + (uint32_t)(j * USTEP);
+                const uint32_t a = PAD ? G::pack(lt, lu, (uint32_t)lv) : lt + TT * (lu + TU * (uint32_t)lv);
+                if (LOAD) tile[a] = glat[base + row[j]];
+                else glat[base + row[j]] = tile[a];
+            }
+            base += plane;
+            base -= (base >= nsite) ? nsite : 0u;
         }
     } else {
         for (int i = tid; i < TT * TU * TV; i += THREADS) {
