@@ -241,23 +241,36 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //   * every LDS read a move needs is issued before the accept/reject test;
 //     DiffuseSL waits for two dependent LDS round trips, TransForw for three
 //     (site, bond-pair table, target) and TransBack for four (site, source,
-//     combine table, target): the two small LDS tables after the tile replace
-//     about 20 VALU instructions per iteration, which measured faster than
-//     the shorter chains.  Accept/reject rules and writes are those of the
+//     combine table, target): the small LDS tables after the tile (bond
+//     pairs, combined bonds and their packed offsets) replace about 30 VALU
+//     instructions per iteration, which measured faster than the shorter
+//     chains.  Accept/reject rules and writes are those of the
 //     functions above.
 // =====================================================================
+constexpr int kCombine = 256;
+// LDS after the tile (ring, with tables): 64 fold pairs, 256 combine codes
+// (bytes), then the packed offsets of the pairs' first bonds (64 words) and
+// of the combined bonds (256 words).
+constexpr int kRingTabBytes = kFoldPairs + kCombine;                  // 320, word aligned
+constexpr int kRingTabTotal = kRingTabBytes + 4 * (kFoldPairs + kCombine);
+
 // PAIRS: read fold_pair from a 64-byte LDS table after the tile
 // (lat[G::NSITE + 4 next + (r & 3)], filled by RingMovesT::init) instead of
-// computing it: one more LDS round trip, about 12 fewer VALU instructions.
+// computing it, and the packed offset of its first bond from a parallel
+// table of 64 words: one more LDS round trip, about 17 fewer VALU
+// instructions, none of them between that read and the target read.
 template <class G, bool PAIRS = false>
 __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);
-    const uint32_t pair = PAIRS ? (uint32_t)lat[G::NSITE + ((next << 2) | (r & 3u))] : fold_pair(next, r);
+    const uint32_t pi = (next << 2) | (r & 3u);
+    const uint32_t pair = PAIRS ? (uint32_t)lat[G::NSITE + pi] : fold_pair(next, r);
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4;
-    const uint32_t Pd = P + G::delta(nb1);
+    // with the tables, the offset of nb1 is read beside the pair instead of computed after it
+    const uint32_t Pd = P + (PAIRS ? reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes)[pi]
+                                   : G::delta(nb1));
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
     uint32_t ns = lat[na];
     const uint32_t ds = lat[da];
@@ -290,7 +303,6 @@ __device__ __forceinline__ uint32_t combine_code(uint32_t next, uint32_t srcNext
     const bool ok = bond_add_valid(next, srcNext, c);
     return bond_add(next, srcNext) | (ok ? 0x10u : 0u);
 }
-constexpr int kCombine = 256;
 
 template <class G, bool TABLE = false>
 __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
@@ -301,9 +313,11 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t sa = G::addr(Ps);
     const uint32_t ss = lat[sa];
     const uint32_t srcNext = ss & 0xfu;
-    const uint32_t cc = TABLE ? (uint32_t)lat[G::NSITE + kFoldPairs + ((next << 4) | srcNext)] : combine_code(next, srcNext);
+    const uint32_t ci = (next << 4) | srcNext;
+    const uint32_t cc = TABLE ? (uint32_t)lat[G::NSITE + kFoldPairs + ci] : combine_code(next, srcNext);
     const uint32_t newNext = cc & 0xfu;
-    const uint32_t Pw = P + G::delta(newNext);
+    const uint32_t Pw =
+        P + (TABLE ? reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes)[kFoldPairs + ci] : G::delta(newNext));
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
@@ -349,14 +363,20 @@ template <bool PAIRS>
 struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
-    static constexpr int EXTRA = PAIRS ? kFoldPairs + kCombine : 0;   // LDS bytes after the tile
+    static constexpr int EXTRA = PAIRS ? kRingTabTotal : 0;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
         if (!PAIRS) return;
         fill_fold_pairs(lat + G::NSITE, tid);
-        if (tid < 64)
-            for (int i = tid; i < kCombine; i += 64)
-                lat[G::NSITE + kFoldPairs + i] = (uint8_t)combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
+        uint32_t* off = reinterpret_cast<uint32_t*>(lat + G::NSITE + kRingTabBytes);
+        if (tid < 64) {
+            off[tid] = G::delta(fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u) & 0xfu);
+            for (int i = tid; i < kCombine; i += 64) {
+                const uint32_t cc = combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
+                lat[G::NSITE + kFoldPairs + i] = (uint8_t)cc;
+                off[kFoldPairs + i] = G::delta(cc & 0xfu);
+            }
+        }
     }
     template <class G>
     __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
@@ -380,7 +400,7 @@ using RingMoves = RingMovesT<false>;
 
 template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p) {
-    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
+    __shared__ __align__(16) uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
     tile_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p),
                                                                   xcd_tile(blockIdx.x, gridDim.x));
 }
@@ -388,7 +408,7 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__
 // One time unit of every replica of a batch (one workgroup per tile).
 template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
 __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__ gbase, BatchLaunch b) {
-    __shared__ uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
+    __shared__ __align__(16) uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
     batch_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
 }
 
