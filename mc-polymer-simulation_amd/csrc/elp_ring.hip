@@ -275,20 +275,16 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
     uint32_t ns = lat[na];
     const uint32_t ds = lat[da];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u;
-    if (!s || G::out(Pn | Pd) || !(sl | nsl) || ds) return;
-    const bool moveFirst = (sl && nsl) ? ((r >> 2) & 1u) : (sl != 0);
-    uint32_t d = nb2;
-    if (moveFirst) {
-        s = nb1 | ((label >> 1) & 0x10u);
-        d |= label & 0x10u;
-    } else {
-        s = nb1 | label | sl;
-        d |= (ns & 0x20u) >> 1;
-        ns &= 0x1fu;
+    // both outcomes as selects, one guarded write block (no early returns)
+    const bool ok = (s != 0u) & !G::out(Pn | Pd) & ((sl | nsl) != 0u) & (ds == 0u);
+    const bool moveFirst = (sl & (nsl >> 0)) ? ((r >> 2) & 1u) : (sl != 0u);
+    const uint32_t s1 = moveFirst ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
+    const uint32_t d1 = nb2 | (moveFirst ? (label & 0x10u) : ((ns & 0x20u) >> 1));
+    if (ok) {
+        lat[a] = (uint8_t)s1;
+        lat[da] = (uint8_t)d1;
+        if (!moveFirst) lat[na] = (uint8_t)(ns & 0x1fu);
     }
-    lat[a] = (uint8_t)s;
-    lat[da] = (uint8_t)d;
-    if (!moveFirst) lat[na] = (uint8_t)ns;
 }
 
 // rules = 1 (balanced): a fold forced by stored length at exactly one end
@@ -321,19 +317,17 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
-    if (!s || G::out(Ps | Pw) || (ss & 0x40u) || !(cc & 0x10u) || (sl && wsl)) return;
-    if (rules && (sl | wsl) && (r & 6u)) return;
-    const bool moveFirst = (!sl && !wsl) ? (r & 1u) : (sl == 0);
+    const bool ok = (s != 0u) & !G::out(Ps | Pw) & ((ss & 0x40u) == 0u) & ((cc & 0x10u) != 0u) & ((sl & wsl) == 0u) &
+                    !((rules != 0u) & ((sl | wsl) != 0u) & ((r & 6u) != 0u));
+    const bool moveFirst = ((sl | wsl) == 0u) ? (r & 1u) : (sl == 0u);
     const uint32_t srcLabel = ss & 0x30u;
-    if (moveFirst) {
-        s = newNext | (label << 1) | srcLabel | 0x40u;
-    } else {
-        s = newNext | label | sl;
-        ws = (ws & 0x3fu) | (srcLabel << 1) | 0x40u;
+    const uint32_t s1 = moveFirst ? (newNext | (label << 1) | srcLabel | 0x40u) : (newNext | label | sl);
+    const uint32_t w1 = moveFirst ? ws : ((ws & 0x3fu) | (srcLabel << 1) | 0x40u);
+    if (ok) {
+        lat[sa] = 0;
+        lat[a] = (uint8_t)s1;
+        lat[wa] = (uint8_t)w1;
     }
-    lat[sa] = 0;
-    lat[a] = (uint8_t)s;
-    lat[wa] = (uint8_t)ws;
 }
 
 template <class G>
@@ -345,16 +339,13 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     const uint32_t na = G::addr(Pn);
     uint32_t ns = lat[na];
     const uint32_t label = s & 0x30u, sl = s & 0x40u;
-    if (!s || G::out(Pn) || ((ns & 0x40u) + sl) != 0x40u) return;
-    if (sl) {
-        ns = ns | ((label & 0x10u) << 1) | 0x40u;
-        s = next | ((label >> 1) & 0x10u);
-    } else {
-        s = next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u;
-        ns &= 0x1fu;
+    const bool ok = (s != 0u) & !G::out(Pn) & (((ns & 0x40u) + sl) == 0x40u);
+    const uint32_t s1 = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u);
+    const uint32_t n1 = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
+    if (ok) {
+        lat[a] = (uint8_t)s1;
+        lat[na] = (uint8_t)n1;
     }
-    lat[a] = (uint8_t)s;
-    lat[na] = (uint8_t)ns;
 }
 
 // ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
