@@ -259,21 +259,19 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
     const uint32_t next = w & 0xfu;
     const uint32_t Pn = P + G::delta(next);
     const uint32_t na = G::addr(Pn);
-    uint32_t x = lat[na];
-    if (!next || G::out(Pn)) return;
-    if ((w & kSL) && !(x & kSL)) {
-        const uint32_t l0 = (w >> 9) & 1u, l1 = (w >> 10) & 1u;
-        w = (w & ~(kL0 | kL1 | kSL)) | (l1 << 9);
-        x |= (l0 << 10) | kSL;
-    } else if (!(w & kSL) && (x & kSL)) {
-        const uint32_t q = (x >> 10) & 1u;
-        x &= ~(kL1 | kSL);
-        w = (w & ~kL0) | (((w >> 9) & 1u) << 10) | (q << 9) | kSL;
-    } else {
-        return;
+    const uint32_t x = lat[na];
+    // both directions as selects, one guarded write block
+    const bool wsl = (w & kSL) != 0u, xsl = (x & kSL) != 0u;
+    const bool toX = wsl & !xsl, toW = !wsl & xsl;
+    const bool ok = (next != 0u) & !G::out(Pn) & (toX | toW);
+    const uint32_t wX = (w & ~(kL0 | kL1 | kSL)) | (((w >> 10) & 1u) << 9);
+    const uint32_t xX = x | (((w >> 9) & 1u) << 10) | kSL;
+    const uint32_t wW = (w & ~kL0) | (((w >> 9) & 1u) << 10) | (((x >> 10) & 1u) << 9) | kSL;
+    const uint32_t xW = x & ~(kL1 | kSL);
+    if (ok) {
+        lat[a] = (uint16_t)(toX ? wX : wW);
+        lat[na] = (uint16_t)(toX ? xX : xW);
     }
-    lat[a] = (uint16_t)w;
-    lat[na] = (uint16_t)x;
 }
 
 // The two sites a case reads are X2 = P + delta(c2) and X1 = base + delta(c1):
@@ -305,60 +303,45 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     const uint32_t X2 = P + G::delta(c2);
     const uint32_t X1 = ((sl && bondS && pbS) ? X2 : P) + G::delta(c1);
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
-    uint32_t x1 = lat[a1];
-    uint32_t x2 = lat[a2];
-    if (!(w & 0xffu)) return;
-    if (sl) {
-        if (bondS == 0u) {   // grow the chain end out of the stored length
-            const uint32_t pb = pbS ^ 1u;
-            if (nb == 0u || nb == 3u || nb == 0xcu || nb == 0xfu || G::out(X1) || (x1 & 0xffu)) return;
-            const uint32_t ins = pb ? nb : (nb ^ 0xfu);
-            const uint32_t lab = (w >> 9) & 3u;
-            w = (w & ~(kL0 | kL1 | kSL)) | (((lab >> pb) & 1u) << 9) | (ins << (4 * (pb ^ 1u)));
-            x1 |= (((lab >> (pb ^ 1u)) & 1u) << 9) | (ins << (4 * pb));
-            lat[a] = (uint16_t)w;
-            lat[a1] = (uint16_t)x1;
-            return;
-        }
-        if (G::out(X2 | X1) || (x1 & 0xffu)) return;   // split the bond through an empty site
-        const uint32_t lab = (w >> (9 + pbS)) & 1u;
-        x1 |= nNext | (nPrev << 4) | (lab << 9);
-        if (pbS) {
-            w = (w & ~(0xf0u | kSL | kL1)) | (nNext << 4);
-            x2 = (x2 & ~0xfu) | nPrev;
-        } else {
-            w = (w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9);
-            x2 = (x2 & ~0xf0u) | (nNext << 4);
-        }
-        lat[a] = (uint16_t)w;
-        lat[a1] = (uint16_t)x1;
-        lat[a2] = (uint16_t)x2;
-        return;
-    }
-    if ((r >> 4) & 1u) return;
-    if (end) {   // retract a chain end onto its neighbour as stored length
-        if (((r >> 5) & 7u) || G::out(X1) || (x1 & kSL)) return;
-        const uint32_t pb = (next == 0u) ? 1u : 0u;
-        const uint32_t nl = (((w >> 9) & 1u) << (pb ^ 1u)) | (((x1 >> 9) & 1u) << pb);
-        w &= ~(kL0 | (0xfu << (4 * pb)));
-        x1 = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pb ^ 1u))))) | (nl << 9) | kSL;
-        lat[a] = (uint16_t)w;
-        lat[a1] = (uint16_t)x1;
-        return;
-    }
+    const uint32_t x1 = lat[a1];
+    const uint32_t x2 = lat[a2];
+    // The four cases as selects and one guarded write block (a wave whose lanes
+    // take different cases then runs one path instead of all of them).
+    const bool occ = (w & 0xffu) != 0u;
+    const bool gate = ((r >> 4) & 1u) == 0u;            // cases without stored length
+    // grow the chain end out of the stored length
+    const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
+    const bool okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
+    const uint32_t pbG = pbS ^ 1u, ins = pbG ? nb : (nb ^ 0xfu), labG = (w >> 9) & 3u;
+    const uint32_t wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
+    const uint32_t x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
+    // split the chosen bond through an empty site
+    const bool okSplit = sl & (bondS != 0u) & !G::out(X2 | X1) & ((x1 & 0xffu) == 0u);
+    const uint32_t labS = (w >> (9 + pbS)) & 1u;
+    const uint32_t x1S = x1 | nNext | (nPrev << 4) | (labS << 9);
+    const uint32_t wS = pbS ? ((w & ~(0xf0u | kSL | kL1)) | (nNext << 4))
+                            : ((w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9));
+    const uint32_t x2S = pbS ? ((x2 & ~0xfu) | nPrev) : ((x2 & ~0xf0u) | (nNext << 4));
+    // retract a chain end onto its neighbour as stored length
+    const bool okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
+    const uint32_t pbE = (next == 0u) ? 1u : 0u;
+    const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
+    const uint32_t wE = w & ~(kL0 | (0xfu << (4 * pbE)));
+    const uint32_t x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
     // fold the monomer onto its prev (r bit 0) or next neighbour as stored length
     uint32_t bond;
-    if (G::out(X1 | X2) || !bond_add_valid(next, prev, bond)) return;
-    const uint32_t pb = r & 1u;
-    if ((pb ? x1 : x2) & kSL) return;
-    const uint32_t lab = (w >> 9) & 1u;
-    if (pb) x1 = (((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (lab << 9));
-    else x2 |= kSL | (lab << 10);
-    x1 = (x1 & ~0xfu) | bond;
-    x2 = (x2 & ~0xf0u) | (bond << 4);
-    lat[a] = (uint16_t)(w & ~(kL0 | 0xffu));
-    lat[a1] = (uint16_t)x1;
-    lat[a2] = (uint16_t)x2;
+    const bool bondOk = bond_add_valid(next, prev, bond);
+    const uint32_t pbF = r & 1u, labF = (w >> 9) & 1u;
+    const bool okFold = !sl & gate & !end & !G::out(X1 | X2) & bondOk & ((((pbF ? x1 : x2)) & kSL) == 0u);
+    const uint32_t x1f = pbF ? (((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (labF << 9)) : x1;
+    const uint32_t x2f = pbF ? x2 : (x2 | kSL | (labF << 10));
+    const uint32_t x1F = (x1f & ~0xfu) | bond, x2F = (x2f & ~0xf0u) | (bond << 4);
+    const uint32_t wF = w & ~(kL0 | 0xffu);
+    if (occ & (okGrow | okSplit | okEnd | okFold)) {
+        lat[a] = (uint16_t)(okGrow ? wG : okSplit ? wS : okEnd ? wE : wF);
+        lat[a1] = (uint16_t)(okGrow ? x1G : okSplit ? x1S : okEnd ? x1E : x1F);
+        if (okSplit | okFold) lat[a2] = (uint16_t)(okSplit ? x2S : x2F);
+    }
 }
 
 // linpol's substep sequence {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490)
