@@ -144,3 +144,14 @@ int elp_ring_host_seeds(const elp_ring_host* h, uint32_t* out, uint64_t n) {
     memcpy(out, h->ss.seeds, need * sizeof(uint32_t));
     return 0;
 }
+
+/* The host generator (gpupol2/gpupol.h:84-95, the LFSR113 step of util/rng.h)
+   from an explicit state: n outputs into out, the state advanced in place.
+   Exposed so that tests can compare it with the efpol restatement's. */
+int elp_host_lfsr113(uint32_t state[4], uint32_t* out, int n) {
+    if (!state || !out || n < 0) return -1;
+    uint4_t s = {state[0], state[1], state[2], state[3]};
+    for (int i = 0; i < n; i++) out[i] = elp_lfsr113(&s);
+    state[0] = s.x; state[1] = s.y; state[2] = s.z; state[3] = s.w;
+    return 0;
+}

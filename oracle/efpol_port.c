@@ -334,6 +334,15 @@ static int bl_for(int L) {
 static double now_s(void) { struct timeval tv; gettimeofday(&tv, NULL); return tv.tv_sec + 1e-6 * tv.tv_usec; }
 
 int main(int argc, char** argv) {
+    if (argc >= 2 && !strcmp(argv[1], "rng")) {
+        /* rng X Y Z W N: N outputs of the generator from the state (X,Y,Z,W), one per line */
+        if (argc < 7) { fprintf(stderr, "usage: %s rng X Y Z W N\n", argv[0]); return 2; }
+        uint32_t s[4];
+        for (int k = 0; k < 4; k++) s[k] = (uint32_t)strtoul(argv[2 + k], NULL, 10);
+        const long n = atol(argv[6]);
+        for (long i = 0; i < n; i++) printf("%u\n", lfsr113(s));
+        return 0;
+    }
     if (argc >= 2 && !strcmp(argv[1], "bench")) {
         if (argc < 7) { fprintf(stderr, "usage: %s bench L NMONO DENSITY SEED NTIME [NWARM]\n", argv[0]); return 2; }
         int L = atoi(argv[2]), N = atoi(argv[3]);
