@@ -348,6 +348,59 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     }
 }
 
+// Balanced rates with the LDS tables: the three moves a slot may draw
+// (r bits 6-7: 0,1 TransForw; 2 DiffuseSL; 3 TransBack) in one pass, so a
+// wave whose lanes drew different moves waits for four dependent LDS round
+// trips instead of the nine of the three functions in turn.  All three read
+// the site and its bond partner (TransForw's n, DiffuseSL's n, TransBack's
+// source); TransForw then reads its target P + nb1 and TransBack its fold
+// site P + (next + srcNext), one read of the drawn one.  The outcome and the
+// writes are those of trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
+template <class G>
+__device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
+    const uint32_t* off = reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes);
+    const uint32_t ty = r >> 6;
+    const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
+    const uint32_t a = G::addr(P);
+    const uint32_t s = lat[a];
+    const uint32_t next = s & 0xfu;
+    const uint32_t Pn = P + G::delta(next);                 // bond partner (TransBack: its source)
+    const uint32_t na = G::addr(Pn);
+    const uint32_t pi = (next << 2) | (r & 3u);
+    const uint32_t pair = lat[G::NSITE + pi];
+    const uint32_t offPd = off[pi];
+    const uint32_t ns = lat[na];
+    const uint32_t ci = (next << 4) | (ns & 0xfu);
+    const uint32_t cc = lat[G::NSITE + kFoldPairs + ci];
+    const uint32_t X = P + (isTB ? off[kFoldPairs + ci] : offPd);
+    const uint32_t xa = G::addr(X);
+    const uint32_t x = lat[xa];
+    const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u, xsl = x & 0x40u;
+    const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4, newNext = cc & 0xfu;
+    // TransForw: an empty target next to both ends, stored length at one of them
+    const bool okTF = isTF & (s != 0u) & !G::out(Pn | X) & ((sl | nsl) != 0u) & (x == 0u);
+    const bool mfTF = (sl & nsl) ? ((r >> 2) & 1u) : (sl != 0u);
+    const uint32_t sTF = mfTF ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
+    const uint32_t xTF = nb2 | (mfTF ? (label & 0x10u) : ((ns & 0x20u) >> 1));
+    // DiffuseSL: stored length at exactly one end
+    const bool okDSL = isDSL & (s != 0u) & !G::out(Pn) & ((nsl + sl) == 0x40u);
+    const uint32_t sDSL = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u);
+    const uint32_t nDSL = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
+    // TransBack: fold the source onto the site or the third corner; a fold
+    // forced by stored length at one end is taken with probability 1/4
+    const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & (nsl == 0u) & ((cc & 0x10u) != 0u) & ((sl & xsl) == 0u) &
+                      !(((sl | xsl) != 0u) & ((r & 6u) != 0u));
+    const bool mfTB = ((sl | xsl) == 0u) ? (r & 1u) : (sl == 0u);
+    const uint32_t srcLabel = ns & 0x30u;
+    const uint32_t sTB = mfTB ? (newNext | (label << 1) | srcLabel | 0x40u) : (newNext | label | sl);
+    const uint32_t xTB = mfTB ? x : ((x & 0x3fu) | (srcLabel << 1) | 0x40u);
+    if (okTF | okDSL | okTB) {
+        lat[a] = (uint8_t)(okTF ? sTF : okDSL ? sDSL : sTB);
+        if (okDSL | okTB | (okTF & !mfTF)) lat[na] = (uint8_t)(okTF ? (ns & 0x1fu) : okDSL ? nDSL : 0u);
+        if (okTF | okTB) lat[xa] = (uint8_t)(okTF ? xTF : xTB);
+    }
+}
+
 // ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
 // (gpupol2/ringpol3.cu:278-292) as the move set of tile_unit.
 template <bool PAIRS>
@@ -376,10 +429,14 @@ struct RingMovesT {
             // byte (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the
             // proportions of the reference sequence), so each substep is a
             // symmetric kernel and preserves the uniform measure by itself
-            const uint32_t ty = r >> 6;
-            if (ty < 2u) trans_forw2<G, PAIRS>(lat, P, r);
-            else if (ty == 2u) diffuse_sl2<G>(lat, P);
-            else trans_back2<G, PAIRS>(lat, P, r, rules);
+            if constexpr (PAIRS) {
+                balanced_move<G>(lat, P, r);
+            } else {
+                const uint32_t ty = r >> 6;
+                if (ty < 2u) trans_forw2<G, PAIRS>(lat, P, r);
+                else if (ty == 2u) diffuse_sl2<G>(lat, P);
+                else trans_back2<G, PAIRS>(lat, P, r, rules);
+            }
             return;
         }
         if (s == 1) diffuse_sl2<G>(lat, P);
