@@ -1,6 +1,6 @@
 """Equilibrium statistics of the GPU ring engine (ring melt L=32, N=100,
-density 1.2, 8 seeds, 40000 time units, frames every 1000, the first 11
-discarded), for the reference schedule (32^3 tiles, cubic sub-lattice, 51
+density 1.2, 8 seeds run as one batch of 8 replicas, 40000 time units,
+frames every 1000, the first 11 discarded), for the reference schedule (32^3 tiles, cubic sub-lattice, 51
 iterations) and the dense schedule (16^3, dense, 13), against each other and
 against efpol (tests/golden/efpol_stats/ring_L32_N100.json, written by the
 script beside it).
@@ -49,23 +49,28 @@ def _run(name, rules=elp.RULES_REFERENCE):
     if (name, rules) in _cache:
         return _cache[(name, rules)]
     tile, sub, n_iter = {**SCHEDULES, **OTHER}[name]
-    res = []
-    for seed in range(8):
-        _say(f"[statistics] {name} rules {rules} seed {seed}")
-        h = elp.RingHost(N=N, L=L, density=REF["density"], seed=seed)
-        eng = elp.Engine("ring", h.L, tile, sub, n_iter)
-        eng.set_rules(rules)
-        eng.upload(h.lattice())
+    # the 8 seeds as one batch of 8 replicas (each bit-exact with its solo run)
+    _say(f"[statistics] {name} rules {rules}, seeds 0..7 as one batch")
+    hosts = [elp.RingHost(N=N, L=L, density=REF["density"], seed=s) for s in range(8)]
+    bat = elp.Batch("ring", [h.L for h in hosts], tile, sub, n_iter)
+    bat.set_rules(rules)
+    series = []
+    for r, h in enumerate(hosts):
+        bat.upload(r, h.lattice())
         h.load_shift(tile)
-        ser = Series(True, h.L, h.n_pol, h.N, T // INTERVAL + 1)
-        ser.add_host(h)
-        tau = 0
+        series.append(Series(True, h.L, h.n_pol, h.N, T // INTERVAL + 1))
+        series[r].add_host(h)
+    tau = 0
+    for _ in range(T // INTERVAL):
+        org = np.stack([h.origins(tile, INTERVAL) for h in hosts], axis=1)
+        bat.run(org, tau0=tau, seeds=[h.seed for h in hosts], streams=[0] * 8, units_per_graph=16)
+        tau += INTERVAL
+        for r, h in enumerate(hosts):
+            h.set_lattice(bat.download(r))
+            series[r].add_host(h)
+    res = []
+    for h, ser in zip(hosts, series):
         stack3 = 0
-        for _ in range(T // INTERVAL):
-            eng.run(h.origins(tile, INTERVAL), tau0=tau, seed=h.seed)
-            tau += INTERVAL
-            h.set_lattice(eng.download())
-            ser.add_host(h)
         for c in range(h.n_pol):
             b = h.polymer(c)[1]
             stack3 += int(np.sum((b == 0) & (np.roll(b, 1) == 0)))
@@ -73,7 +78,7 @@ def _run(name, rules=elp.RULES_REFERENCE):
         g1, g3 = ser.msd(N_THERM, max(LAGS) + 1)
         res.append({"rg2": st["rg2"], "slrat": st["slrat"], "pc": st["pc"], "g1": g1[LAGS], "g3": g3[LAGS],
                     "stack3": stack3})
-        eng.close()
+    bat.close()
     _cache[(name, rules)] = res
     return res
 
