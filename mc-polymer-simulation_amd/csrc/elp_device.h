@@ -66,7 +66,7 @@ __device__ __forceinline__ uint32_t trans_first(uint32_t next, uint32_t k) {
 // The ordered bond pair (nb1, nb2) that replaces bond `next` for random bits
 // r & 3: nb1 = trans_first(next, r & 1), nb2 = the bond with nb1 + nb2 = next,
 // swapped when bit 1 is clear; returned as nb1 | nb2 << 4.  The move kernels
-// read it from a 64-entry LDS table, kFoldPairs bytes after the tile.
+// read it from a 64-entry LDS table after the tile (entry 4 next + (r & 3)).
 __device__ __forceinline__ uint32_t fold_pair(uint32_t next, uint32_t r) {
     uint32_t nb1 = trans_first(next, r & 1u);
     uint32_t nb2 = bond_add((~nb1) & 0xfu, next);
@@ -74,9 +74,6 @@ __device__ __forceinline__ uint32_t fold_pair(uint32_t next, uint32_t r) {
     return nb1 | (nb2 << 4);
 }
 constexpr int kFoldPairs = 64;
-__device__ __forceinline__ void fill_fold_pairs(uint8_t* table, int tid) {
-    if (tid < kFoldPairs) table[tid] = (uint8_t)fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
-}
 
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
@@ -92,7 +89,11 @@ struct PTile {
     static_assert((1 << LT) == TT && (1 << LU) == TU && (1 << LV) == TV, "tile edges must be powers of two");
     static constexpr uint32_t SU = LT + 1, SV = LT + LU + 2;
     static constexpr uint32_t ONES = 1u | (1u << SU) | (1u << SV);
-    static constexpr uint32_t GUARD = (uint32_t)TT | ((uint32_t)TU << SU) | (~0u << (SV + LV));
+    // The guard stops at bit 23, so bits 24-31 of P are never read: a move
+    // may add an LDS table word that carries a table byte there (an offset
+    // in bits 0-23) to P directly.  An underflow still sets bits SV+LV..23.
+    static_assert(SV + LV < 24, "packed coordinates must fit in 24 bits");
+    static constexpr uint32_t GUARD = ((uint32_t)TT | ((uint32_t)TU << SU) | (~0u << (SV + LV))) & 0xFFFFFFu;
     static constexpr uint32_t MASK = (uint32_t)(TT - 1) | ((uint32_t)(TU - 1) << SU) | ((uint32_t)(TV - 1) << SV);
     static constexpr int NSITE = PAD ? (1 << (SV + LV)) : TT * TU * TV;
     __device__ static __forceinline__ uint32_t pack(uint32_t t, uint32_t u, uint32_t v) { return t | (u << SU) | (v << SV); }

@@ -281,7 +281,9 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
 //   retract (no stored length, chain end): X1 = X2 = E;
 //   fold    (no stored length, interior): X1 = Pp, X2 = Pn.
 // Only the two offsets of the taken case are computed, and the bond pair of
-// the split comes from the LDS table after the tile (fold_pair).
+// the split comes from the LDS table after the tile: word 4 bondS + (r & 3)
+// holds fold_pair in bits 24-31 above the packed offset of its first bond
+// (nPrev) in bits 0-23, so the split's X1 is one add of that word.
 template <class G>
 __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
@@ -292,16 +294,18 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     const uint32_t pbS = (r >> 2) & 1u;
     const uint32_t bondS = pbS ? prev : next;
     const uint32_t nb = (r >> 3) & 0xfu;
-    const uint32_t pair = reinterpret_cast<const uint8_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
+    const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
+    const uint32_t pair = wp >> 24;
     const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
     // cases without stored length: retract an end, or fold an interior monomer
     const bool end = (next == 0u) || (prev == 0u);
     const uint32_t bondE = (next == 0u) ? (prev ^ 0xfu) : next;
     uint32_t c1, c2;
-    if (sl) { c2 = bondS ? (pbS ? (bondS ^ 0xfu) : bondS) : nb; c1 = bondS ? nPrev : nb; }
+    if (sl) { c2 = bondS ? (pbS ? (bondS ^ 0xfu) : bondS) : nb; c1 = nb; }
     else { c2 = end ? bondE : next; c1 = end ? bondE : (prev ^ 0xfu); }
     const uint32_t X2 = P + G::delta(c2);
-    const uint32_t X1 = ((sl && bondS && pbS) ? X2 : P) + G::delta(c1);
+    // split: X1 = (pbS ? X2 : P) + delta(nPrev), the offset taken from the table word
+    const uint32_t X1 = (sl && bondS) ? (pbS ? X2 : P) + wp : P + G::delta(c1);
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     const uint32_t x1 = lat[a1];
     const uint32_t x2 = lat[a2];
@@ -349,10 +353,13 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
 struct LinMoves {
     using site_t = uint16_t;
     static constexpr int NSUB = 2;
-    static constexpr int EXTRA = kFoldPairs;   // LDS bytes after the tile
+    static constexpr int EXTRA = 4 * kFoldPairs;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint16_t* lat, int tid) {
-        fill_fold_pairs(reinterpret_cast<uint8_t*>(lat + G::NSITE), tid);
+        if (tid < kFoldPairs) {
+            const uint32_t pair = fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
+            reinterpret_cast<uint32_t*>(lat + G::NSITE)[tid] = (G::delta(pair & 0xfu) & 0xFFFFFFu) | (pair << 24);
+        }
     }
     template <class G>
     __device__ static __forceinline__ void move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {

@@ -239,26 +239,34 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //   * the LDS tile is plain t-fastest (PAD=0: index rebuilt from P in three
 //     bit-field operations) or indexed by P itself (PAD=1, 16^3 tiles only),
 //   * every LDS read a move needs is issued before the accept/reject test;
-//     DiffuseSL waits for two dependent LDS round trips, TransForw for three
-//     (site, bond-pair table, target) and TransBack for four (site, source,
-//     combine table, target): the small LDS tables after the tile (bond
-//     pairs, combined bonds and their packed offsets) replace about 30 VALU
-//     instructions per iteration, which measured faster than the shorter
-//     chains.  Accept/reject rules and writes are those of the
-//     functions above.
+//     DiffuseSL and TransForw (site, bond-pair table word, target) wait for
+//     two and three dependent LDS round trips, TransBack for three (site,
+//     source, then the combine table beside the target): the small LDS
+//     tables after the tile replace about 30 VALU instructions per
+//     iteration.  Accept/reject rules and writes are those of the functions
+//     above.
 // =====================================================================
 constexpr int kCombine = 256;
-// LDS after the tile (ring, with tables): 64 fold pairs, 256 combine codes
-// (bytes), then the packed offsets of the pairs' first bonds (64 words) and
-// of the combined bonds (256 words).
-constexpr int kRingTabBytes = kFoldPairs + kCombine;                  // 320, word aligned
-constexpr int kRingTabTotal = kRingTabBytes + 4 * (kFoldPairs + kCombine);
+// LDS after the tile (ring, with tables): 64 words, the fold pair of
+// (next, r & 3) in bits 24-31 above the packed offset of its first bond in
+// bits 0-23 (one read gives both; P + word is the target, bits 24-31 being
+// outside every guard and address mask), then 256 bytes, the combine code
+// of (next, srcNext).
+constexpr int kRingTabTotal = 4 * kFoldPairs + kCombine;
 
-// PAIRS: read fold_pair from a 64-byte LDS table after the tile
-// (lat[G::NSITE + 4 next + (r & 3)], filled by RingMovesT::init) instead of
-// computing it, and the packed offset of its first bond from a parallel
-// table of 64 words: one more LDS round trip, about 17 fewer VALU
-// instructions, none of them between that read and the target read.
+template <class G>
+__device__ __forceinline__ uint32_t* ring_tab(uint8_t* lat) {
+    return reinterpret_cast<uint32_t*>(lat + G::NSITE);
+}
+template <class G>
+__device__ __forceinline__ uint8_t* ring_combine_tab(uint8_t* lat) {
+    return lat + G::NSITE + 4 * kFoldPairs;
+}
+
+// PAIRS: read fold_pair and the packed offset of its first bond as one word
+// of the LDS table (word 4 next + (r & 3), filled by RingMovesT::init)
+// instead of computing them: about 17 fewer VALU instructions, none of them
+// between that read and the target read.
 template <class G, bool PAIRS = false>
 __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
@@ -266,11 +274,16 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);
     const uint32_t pi = (next << 2) | (r & 3u);
-    const uint32_t pair = PAIRS ? (uint32_t)lat[G::NSITE + pi] : fold_pair(next, r);
+    uint32_t pair, Pd;
+    if (PAIRS) {
+        const uint32_t w = ring_tab<G>(lat)[pi];
+        pair = w >> 24;
+        Pd = P + w;
+    } else {
+        pair = fold_pair(next, r);
+        Pd = P + G::delta(pair & 0xfu);
+    }
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4;
-    // with the tables, the offset of nb1 is read beside the pair instead of computed after it
-    const uint32_t Pd = P + (PAIRS ? reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes)[pi]
-                                   : G::delta(nb1));
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
     uint32_t ns = lat[na];
     const uint32_t ds = lat[da];
@@ -292,8 +305,10 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
 // rate of the TransForw that undoes it, so every move and its reverse have
 // equal probabilities (with the per-slot move choice of RingMoves::move).
 // Bond next + srcNext as one bond: bits 0-3 the combined bond, bit 4 set when
-// it is a bond (bond_add_valid).  TABLE: read it from a 256-entry LDS table
-// after the fold pairs (lat[G::NSITE + kFoldPairs + 16 next + srcNext]).
+// it is a bond (bond_add_valid).  TABLE: read it from the 256-byte LDS table
+// (entry 16 next + srcNext).  When it is a bond its displacement is the sum
+// of the two (checked for all 256 pairs), so the fold target P + combined
+// bond is Ps + delta(srcNext): read beside the code, not after it.
 __device__ __forceinline__ uint32_t combine_code(uint32_t next, uint32_t srcNext) {
     uint32_t c;
     const bool ok = bond_add_valid(next, srcNext, c);
@@ -309,11 +324,9 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t sa = G::addr(Ps);
     const uint32_t ss = lat[sa];
     const uint32_t srcNext = ss & 0xfu;
-    const uint32_t ci = (next << 4) | srcNext;
-    const uint32_t cc = TABLE ? (uint32_t)lat[G::NSITE + kFoldPairs + ci] : combine_code(next, srcNext);
+    const uint32_t cc = TABLE ? (uint32_t)ring_combine_tab<G>(lat)[(next << 4) | srcNext] : combine_code(next, srcNext);
+    const uint32_t Pw = Ps + G::delta(srcNext);
     const uint32_t newNext = cc & 0xfu;
-    const uint32_t Pw =
-        P + (TABLE ? reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes)[kFoldPairs + ci] : G::delta(newNext));
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
@@ -350,15 +363,16 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
 
 // Balanced rates with the LDS tables: the three moves a slot may draw
 // (r bits 6-7: 0,1 TransForw; 2 DiffuseSL; 3 TransBack) in one pass, so a
-// wave whose lanes drew different moves waits for four dependent LDS round
-// trips instead of the nine of the three functions in turn.  All three read
-// the site and its bond partner (TransForw's n, DiffuseSL's n, TransBack's
-// source); TransForw then reads its target P + nb1 and TransBack its fold
-// site P + (next + srcNext), one read of the drawn one.  The outcome and the
-// writes are those of trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
+// wave whose lanes drew different moves waits for three dependent LDS round
+// trips instead of the eight of the three functions in turn.  All three read
+// the site, then its bond partner (TransForw's n, DiffuseSL's n, TransBack's
+// source) beside the fold-pair word; then TransForw reads its target P + nb1
+// and TransBack its fold site Pn + srcNext, one read of the drawn one,
+// beside the combine code.  The outcome and the writes are those of
+// trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
 template <class G>
 __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
-    const uint32_t* off = reinterpret_cast<const uint32_t*>(lat + G::NSITE + kRingTabBytes);
+    const uint32_t* tab = ring_tab<G>(lat);
     const uint32_t ty = r >> 6;
     const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
     const uint32_t a = G::addr(P);
@@ -366,13 +380,12 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);                 // bond partner (TransBack: its source)
     const uint32_t na = G::addr(Pn);
-    const uint32_t pi = (next << 2) | (r & 3u);
-    const uint32_t pair = lat[G::NSITE + pi];
-    const uint32_t offPd = off[pi];
+    const uint32_t wf = tab[(next << 2) | (r & 3u)];        // fold pair | offset of its first bond
     const uint32_t ns = lat[na];
-    const uint32_t ci = (next << 4) | (ns & 0xfu);
-    const uint32_t cc = lat[G::NSITE + kFoldPairs + ci];
-    const uint32_t X = P + (isTB ? off[kFoldPairs + ci] : offPd);
+    const uint32_t cc = ring_combine_tab<G>(lat)[(next << 4) | (ns & 0xfu)];
+    const uint32_t pair = wf >> 24;
+    // TransBack's fold site P + (next + srcNext) = Pn + delta(srcNext), read beside the combine code
+    const uint32_t X = isTB ? Pn + G::delta(ns & 0xfu) : P + wf;
     const uint32_t xa = G::addr(X);
     const uint32_t x = lat[xa];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u, xsl = x & 0x40u;
@@ -411,15 +424,11 @@ struct RingMovesT {
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
         if (!PAIRS) return;
-        fill_fold_pairs(lat + G::NSITE, tid);
-        uint32_t* off = reinterpret_cast<uint32_t*>(lat + G::NSITE + kRingTabBytes);
-        if (tid < 64) {
-            off[tid] = G::delta(fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u) & 0xfu);
-            for (int i = tid; i < kCombine; i += 64) {
-                const uint32_t cc = combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
-                lat[G::NSITE + kFoldPairs + i] = (uint8_t)cc;
-                off[kFoldPairs + i] = G::delta(cc & 0xfu);
-            }
+        if (tid < kFoldPairs) {
+            const uint32_t pair = fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
+            ring_tab<G>(lat)[tid] = (G::delta(pair & 0xfu) & 0xFFFFFFu) | (pair << 24);
+            for (int i = tid; i < kCombine; i += kFoldPairs)
+                ring_combine_tab<G>(lat)[i] = (uint8_t)combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
         }
     }
     template <class G>
