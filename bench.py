@@ -36,6 +36,18 @@ sys.path.insert(0, PKG)
 HBM_PEAK_GBS = 8000.0
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline_reference(L, N, density, cores, warm, timed, ring=True):
     """The reference efpol itself (cpupol/, built from its sources into oracle/_ref by
     oracle/Makefile.ref): `cores` independent runs of warm + timed time units with a write
@@ -62,6 +74,9 @@ def cpu_baseline_reference(L, N, density, cores, warm, timed, ring=True):
         "value": rate,
         "unit": "attempts/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
+        # efpol's time unit is one attempt per monomer (cpupol/efpol.c:92-94)
+        "monomer_time_units_per_s": rate,
         "kind": "reference",
         "sample": (f"reference efpol (cpupol, built from its sources), {'ring' if ring else 'linear'} L={L} N={N} "
                    f"density {density}, {cores} independent processes (seeds 2000, 2002, ...), {warm} untimed + "
@@ -88,6 +103,8 @@ def cpu_baseline(L, N, density, cores, warm, timed, ring=True):
         "value": rate,
         "unit": "attempts/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
+        "monomer_time_units_per_s": rate,
         "kind": "port",
         "sample": (f"efpol restatement, {'ring' if ring else 'linear'} L={L} N={N} density {density}, "
                    f"{cores} independent processes "
@@ -386,6 +403,9 @@ def main():
     cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     cpu = None if args.no_cpu else cpu_baseline(args.L, N, args.density, cores, warm=300, timed=100, ring=ring)
 
+    # GPU attempts cover every site, empty ones included; normalised like efpol's
+    # rate, a time unit of the whole melt counts once per monomer
+    monomer_rate = host.n_pol * host.N * args.steps / wall
     rec = {
         "metric": "MC move attempts/s, %HBM roofline; equil. wall-time learned vs plain",
         "value": value,
@@ -394,6 +414,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
+        "time_units_per_s": args.steps / wall,
+        "monomer_time_units_per_s": monomer_rate,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -428,6 +450,7 @@ def main():
     }
     if cpu:
         rec["gpu_over_cpu"] = value / cpu["value"]
+        rec["gpu_over_cpu_monomer_time_units"] = monomer_rate / cpu["monomer_time_units_per_s"]
     print(json.dumps(rec))
     return 0
 

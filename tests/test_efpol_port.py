@@ -33,3 +33,17 @@ def test_efpol_port_bench_mode():
                          text=True).stdout
     kv = dict(x.split("=") for x in out.split())
     assert int(kv["attempts"]) == 20 * 32 * 154 and float(kv["seconds"]) > 0
+
+
+def test_bench_cpu_baseline_record():
+    """bench.py's CPU leg on a tiny melt: a rate, the cores and CPU model used, and
+    the monomer-normalised rate (efpol's time unit is one attempt per monomer)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    rec = bench.cpu_baseline(32, 100, 1.2, cores=1, warm=20, timed=10)
+    assert rec is not None and rec["unit"] == "attempts/s" and rec["cores"] == 1 and rec["value"] > 0
+    assert rec["kind"] in ("reference", "port")
+    assert rec["monomer_time_units_per_s"] == rec["value"]
+    assert "cpu_model" in rec
