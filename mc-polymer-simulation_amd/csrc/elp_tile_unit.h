@@ -129,15 +129,19 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     // HBM -> LDS (periodic wrap; origins are < one tile edge, so one subtraction suffices)
     tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
 
-    // slot k of this thread: cell corner (+ j along t for the dense cosets)
-    uint32_t base[SPT], j3[SPT], j2[SPT], slot[SPT];
+    // slot k of this thread: cell corner (+ j along t for the dense cosets).
+    // Dense coset j of substep offset c sits at corner + (j, (c&3) + 3j,
+    // (c>>2) + 2j) mod 4: the u term is (cu + ju) masked to two bits, and
+    // adding 2j mod 4 flips bit 1 for odd j, so the v term is an XOR of the
+    // tile-uniform c>>2 into a corner that already holds that flip (vb).
+    uint32_t base[SPT], ju[SPT], vb[SPT], slot[SPT];
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
         slot[k] = (uint32_t)(tid + k * THREADS);
         const uint32_t cell = slot[k] / SPC, j = slot[k] % SPC;
         base[k] = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
-        j3[k] = 3u * j;
-        j2[k] = 2u * j;
+        ju[k] = (3u * j) << G::SU;
+        vb[k] = base[k] | ((2u * (j & 1u)) << G::SV);
     }
     M::template init<G>(tile, tid);
     __syncthreads();
@@ -157,10 +161,11 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
 #pragma unroll
             for (int s = 0; s < M::NSUB; s++) {
                 const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+                const uint32_t cu = (c & 3u) << G::SU, cv = (c >> 2) << G::SV;
 #pragma unroll
                 for (int k = 0; k < SPT; k++) {
                     uint32_t P;
-                    if (DENSE) P = base[k] + (((c + j3[k]) & 3u) << G::SU) + ((((c >> 2) + j2[k]) & 3u) << G::SV);
+                    if (DENSE) P = ((cu + ju[k]) & (3u << G::SU)) | (vb[k] ^ cv);
                     else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
                     M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu, p.rules);
                 }
