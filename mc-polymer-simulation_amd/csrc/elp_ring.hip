@@ -247,41 +247,51 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit(uint8_t* __restrict__ 
 //     above.
 // =====================================================================
 constexpr int kCombine = 256;
-// LDS after the tile (ring, with tables): 64 words, the fold pair of
-// (next, r & 3) in bits 24-31 above the packed offset of its first bond in
-// bits 0-23 (one read gives both; P + word is the target, bits 24-31 being
-// outside every guard and address mask), then 256 bytes, the combine code
-// of (next, srcNext).
-constexpr int kRingTabTotal = 4 * kFoldPairs + kCombine;
+// LDS after the tile (ring, with tables): 64 entries of two words for
+// (next, r & 3): word 0 the fold pair in bits 24-31 above the packed offset
+// of its first bond in bits 0-23 (P + word is the target, bits 24-31 being
+// outside every guard and address mask), word 1 the packed offset of next
+// itself; then 256 bytes, the combine code of (next, srcNext).
+constexpr int kRingTabTotal = 8 * kFoldPairs + kCombine;
 
 template <class G>
-__device__ __forceinline__ uint32_t* ring_tab(uint8_t* lat) {
-    return reinterpret_cast<uint32_t*>(lat + G::NSITE);
+__device__ __forceinline__ uint2* ring_tab(uint8_t* lat) {
+    return reinterpret_cast<uint2*>(lat + G::NSITE);
 }
 template <class G>
 __device__ __forceinline__ uint8_t* ring_combine_tab(uint8_t* lat) {
-    return lat + G::NSITE + 4 * kFoldPairs;
+    return lat + G::NSITE + 8 * kFoldPairs;
 }
 
-// PAIRS: read fold_pair and the packed offset of its first bond as one word
-// of the LDS table (word 4 next + (r & 3), filled by RingMovesT::init)
-// instead of computing them: about 17 fewer VALU instructions, none of them
-// between that read and the target read.
+// moveFirst of TransForw, (sl & nsl) ? r bit 2 : sl != 0, as one bit test
+// (checked for all s, ns < 128 and r < 256).
+__device__ __forceinline__ bool forw_move_first(uint32_t s, uint32_t ns, uint32_t r) {
+    return ((s & (~ns | (r << 4))) & 0x40u) != 0u;
+}
+
+// PAIRS: read fold_pair, the packed offset of its first bond and that of
+// next as one 64-bit entry of the LDS table (entry 4 next + (r & 3), filled
+// by RingMovesT::init) instead of computing them: about 22 fewer VALU
+// instructions; the bond partner is read beside the target, so the move
+// still waits for three dependent LDS reads.  An empty site needs no test of
+// its own: next = 0 puts the partner on the site itself, and neither end
+// then holds stored length.
 template <class G, bool PAIRS = false>
 __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
-    const uint32_t Pn = P + G::delta(next);
     const uint32_t pi = (next << 2) | (r & 3u);
-    uint32_t pair, Pd;
+    uint32_t pair, Pd, Pn;
     if (PAIRS) {
-        const uint32_t w = ring_tab<G>(lat)[pi];
-        pair = w >> 24;
-        Pd = P + w;
+        const uint2 w = ring_tab<G>(lat)[pi];
+        pair = w.x >> 24;
+        Pd = P + w.x;
+        Pn = P + w.y;
     } else {
         pair = fold_pair(next, r);
         Pd = P + G::delta(pair & 0xfu);
+        Pn = P + G::delta(next);
     }
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4;
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
@@ -289,8 +299,8 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t ds = lat[da];
     const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u;
     // both outcomes as selects, one guarded write block (no early returns)
-    const bool ok = (s != 0u) & !G::out(Pn | Pd) & ((sl | nsl) != 0u) & (ds == 0u);
-    const bool moveFirst = (sl & (nsl >> 0)) ? ((r >> 2) & 1u) : (sl != 0u);
+    const bool ok = !G::out(Pn | Pd) & ((sl | nsl) != 0u) & (ds == 0u);
+    const bool moveFirst = forw_move_first(s, ns, r);
     const uint32_t s1 = moveFirst ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
     const uint32_t d1 = nb2 | (moveFirst ? (label & 0x10u) : ((ns & 0x20u) >> 1));
     if (ok) {
@@ -352,7 +362,8 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     const uint32_t na = G::addr(Pn);
     uint32_t ns = lat[na];
     const uint32_t label = s & 0x30u, sl = s & 0x40u;
-    const bool ok = (s != 0u) & !G::out(Pn) & (((ns & 0x40u) + sl) == 0x40u);
+    // an empty site (next = 0) reads itself as partner: no stored length, no move
+    const bool ok = !G::out(Pn) & (((ns & 0x40u) + sl) == 0x40u);
     const uint32_t s1 = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u);
     const uint32_t n1 = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
     if (ok) {
@@ -372,7 +383,7 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
 // trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
 template <class G>
 __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
-    const uint32_t* tab = ring_tab<G>(lat);
+    const uint2* tab = ring_tab<G>(lat);
     const uint32_t ty = r >> 6;
     const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
     const uint32_t a = G::addr(P);
@@ -380,7 +391,7 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);                 // bond partner (TransBack: its source)
     const uint32_t na = G::addr(Pn);
-    const uint32_t wf = tab[(next << 2) | (r & 3u)];        // fold pair | offset of its first bond
+    const uint32_t wf = tab[(next << 2) | (r & 3u)].x;      // fold pair | offset of its first bond
     const uint32_t ns = lat[na];
     const uint32_t cc = ring_combine_tab<G>(lat)[(next << 4) | (ns & 0xfu)];
     const uint32_t pair = wf >> 24;
@@ -392,7 +403,7 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4, newNext = cc & 0xfu;
     // TransForw: an empty target next to both ends, stored length at one of them
     const bool okTF = isTF & (s != 0u) & !G::out(Pn | X) & ((sl | nsl) != 0u) & (x == 0u);
-    const bool mfTF = (sl & nsl) ? ((r >> 2) & 1u) : (sl != 0u);
+    const bool mfTF = forw_move_first(s, ns, r);
     const uint32_t sTF = mfTF ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
     const uint32_t xTF = nb2 | (mfTF ? (label & 0x10u) : ((ns & 0x20u) >> 1));
     // DiffuseSL: stored length at exactly one end
@@ -426,7 +437,8 @@ struct RingMovesT {
         if (!PAIRS) return;
         if (tid < kFoldPairs) {
             const uint32_t pair = fold_pair((uint32_t)tid >> 2, (uint32_t)tid & 3u);
-            ring_tab<G>(lat)[tid] = (G::delta(pair & 0xfu) & 0xFFFFFFu) | (pair << 24);
+            ring_tab<G>(lat)[tid] = make_uint2((G::delta(pair & 0xfu) & 0xFFFFFFu) | (pair << 24),
+                                               G::delta((uint32_t)tid >> 2));
             for (int i = tid; i < kCombine; i += kFoldPairs)
                 ring_combine_tab<G>(lat)[i] = (uint8_t)combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
         }
