@@ -263,6 +263,12 @@ __device__ __forceinline__ uint8_t* ring_combine_tab(uint8_t* lat) {
     return lat + G::NSITE + 8 * kFoldPairs;
 }
 
+// a where the mask m is set, b elsewhere (one v_bfi_b32): lets a move pick
+// between its two outcomes without a divergent branch.
+__device__ __forceinline__ uint32_t bitsel(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
+// all ones when bit 6 (stored length) of x is set
+__device__ __forceinline__ uint32_t bit6_mask(uint32_t x) { return (uint32_t)((int32_t)(x << 25) >> 31); }
+
 // moveFirst of TransForw, (sl & nsl) ? r bit 2 : sl != 0, as one bit test
 // (checked for all s, ns < 128 and r < 256).
 __device__ __forceinline__ bool forw_move_first(uint32_t s, uint32_t ns, uint32_t r) {
@@ -325,6 +331,36 @@ __device__ __forceinline__ uint32_t combine_code(uint32_t next, uint32_t srcNext
     return bond_add(next, srcNext) | (ok ? 0x10u : 0u);
 }
 
+// TransBack's tests on bit 6 (stored length) of the site s, its source ss and
+// the fold target ws, with the combine code cc (checked for every combination
+// of those bits):
+//   back_ok_bits: the combined bond is valid, the source holds no stored
+//     length, and not both the site and the target hold it;
+//   back_move_first: (sl | wsl) == 0 ? r bit 0 : sl == 0, as a mask.
+__device__ __forceinline__ bool back_ok_bits(uint32_t s, uint32_t ss, uint32_t ws, uint32_t cc) {
+    return (((cc << 2) & ~ss & ~(s & ws)) & 0x40u) != 0u;
+}
+__device__ __forceinline__ uint32_t back_move_first(uint32_t s, uint32_t ws, uint32_t r) {
+    return bit6_mask(~s & (ws | (r << 6)));
+}
+// TransBack's new site and fold-target bytes: the source monomer joins the
+// site (moveFirst) or the fold target.
+__device__ __forceinline__ uint32_t back_site(uint32_t mf, uint32_t s, uint32_t ss, uint32_t newNext) {
+    return newNext | bitsel(mf, ((s & 0x30u) << 1) | (ss & 0x30u) | 0x40u, s & 0x70u);
+}
+__device__ __forceinline__ uint32_t back_target(uint32_t mf, uint32_t ss, uint32_t ws) {
+    return bitsel(mf, ws, (ws & 0x3fu) | ((ss & 0x30u) << 1) | 0x40u);
+}
+// DiffuseSL's new site and partner bytes, m = bit6_mask(s): the stored-length
+// monomer crosses the bond towards the partner (m set) or onto the site.
+__device__ __forceinline__ uint32_t diffuse_site(uint32_t m, uint32_t s, uint32_t ns) {
+    const uint32_t next = s & 0xfu;
+    return bitsel(m, next | ((s >> 1) & 0x10u), next | ((s << 1) & 0x20u) | ((ns >> 1) & 0x10u) | 0x40u);
+}
+__device__ __forceinline__ uint32_t diffuse_partner(uint32_t m, uint32_t s, uint32_t ns) {
+    return bitsel(m, ns | ((s & 0x10u) << 1) | 0x40u, ns & 0x1fu);
+}
+
 template <class G, bool TABLE = false>
 __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
     const uint32_t a = G::addr(P);
@@ -339,13 +375,12 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t newNext = cc & 0xfu;
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
-    const uint32_t label = s & 0x30u, sl = s & 0x40u, wsl = ws & 0x40u;
-    const bool ok = (s != 0u) & !G::out(Ps | Pw) & ((ss & 0x40u) == 0u) & ((cc & 0x10u) != 0u) & ((sl & wsl) == 0u) &
+    const uint32_t sl = s & 0x40u, wsl = ws & 0x40u;
+    const bool ok = (s != 0u) & !G::out(Ps | Pw) & back_ok_bits(s, ss, ws, cc) &
                     !((rules != 0u) & ((sl | wsl) != 0u) & ((r & 6u) != 0u));
-    const bool moveFirst = ((sl | wsl) == 0u) ? (r & 1u) : (sl == 0u);
-    const uint32_t srcLabel = ss & 0x30u;
-    const uint32_t s1 = moveFirst ? (newNext | (label << 1) | srcLabel | 0x40u) : (newNext | label | sl);
-    const uint32_t w1 = moveFirst ? ws : ((ws & 0x3fu) | (srcLabel << 1) | 0x40u);
+    const uint32_t mf = back_move_first(s, ws, r);
+    const uint32_t s1 = back_site(mf, s, ss, newNext);
+    const uint32_t w1 = back_target(mf, ss, ws);
     if (ok) {
         lat[sa] = 0;
         lat[a] = (uint8_t)s1;
@@ -361,11 +396,12 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     const uint32_t Pn = P + G::delta(next);
     const uint32_t na = G::addr(Pn);
     uint32_t ns = lat[na];
-    const uint32_t label = s & 0x30u, sl = s & 0x40u;
-    // an empty site (next = 0) reads itself as partner: no stored length, no move
-    const bool ok = !G::out(Pn) & (((ns & 0x40u) + sl) == 0x40u);
-    const uint32_t s1 = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u);
-    const uint32_t n1 = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
+    // stored length at exactly one end; an empty site (next = 0) reads itself
+    // as partner, so neither end holds it and nothing moves
+    const bool ok = !G::out(Pn) & (((ns ^ s) & 0x40u) != 0u);
+    const uint32_t m = bit6_mask(s);
+    const uint32_t s1 = diffuse_site(m, s, ns);
+    const uint32_t n1 = diffuse_partner(m, s, ns);
     if (ok) {
         lat[a] = (uint8_t)s1;
         lat[na] = (uint8_t)n1;
@@ -402,22 +438,22 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u, xsl = x & 0x40u;
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4, newNext = cc & 0xfu;
     // TransForw: an empty target next to both ends, stored length at one of them
-    const bool okTF = isTF & (s != 0u) & !G::out(Pn | X) & ((sl | nsl) != 0u) & (x == 0u);
+    const bool okTF = isTF & !G::out(Pn | X) & ((sl | nsl) != 0u) & (x == 0u);
     const bool mfTF = forw_move_first(s, ns, r);
     const uint32_t sTF = mfTF ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
     const uint32_t xTF = nb2 | (mfTF ? (label & 0x10u) : ((ns & 0x20u) >> 1));
     // DiffuseSL: stored length at exactly one end
-    const bool okDSL = isDSL & (s != 0u) & !G::out(Pn) & ((nsl + sl) == 0x40u);
-    const uint32_t sDSL = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | (((ns & 0x30u) >> 1) & 0x10u) | 0x40u);
-    const uint32_t nDSL = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
+    const bool okDSL = isDSL & !G::out(Pn) & (((ns ^ s) & 0x40u) != 0u);
+    const uint32_t mDSL = bit6_mask(s);
+    const uint32_t sDSL = diffuse_site(mDSL, s, ns);
+    const uint32_t nDSL = diffuse_partner(mDSL, s, ns);
     // TransBack: fold the source onto the site or the third corner; a fold
     // forced by stored length at one end is taken with probability 1/4
-    const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & (nsl == 0u) & ((cc & 0x10u) != 0u) & ((sl & xsl) == 0u) &
+    const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & back_ok_bits(s, ns, x, cc) &
                       !(((sl | xsl) != 0u) & ((r & 6u) != 0u));
-    const bool mfTB = ((sl | xsl) == 0u) ? (r & 1u) : (sl == 0u);
-    const uint32_t srcLabel = ns & 0x30u;
-    const uint32_t sTB = mfTB ? (newNext | (label << 1) | srcLabel | 0x40u) : (newNext | label | sl);
-    const uint32_t xTB = mfTB ? x : ((x & 0x3fu) | (srcLabel << 1) | 0x40u);
+    const uint32_t mfTB = back_move_first(s, x, r);
+    const uint32_t sTB = back_site(mfTB, s, ns, newNext);
+    const uint32_t xTB = back_target(mfTB, ns, x);
     if (okTF | okDSL | okTB) {
         lat[a] = (uint8_t)(okTF ? sTF : okDSL ? sDSL : sTB);
         if (okDSL | okTB | (okTF & !mfTF)) lat[na] = (uint8_t)(okTF ? (ns & 0x1fu) : okDSL ? nDSL : 0u);
