@@ -298,47 +298,61 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     const uint32_t pair = wp >> 24;
     const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
     // cases without stored length: retract an end, or fold an interior monomer
-    const bool end = (next == 0u) || (prev == 0u);
+    const bool end = (next == 0u) | (prev == 0u);
     const uint32_t bondE = (next == 0u) ? (prev ^ 0xfu) : next;
-    uint32_t c1, c2;
-    if (sl) { c2 = bondS ? (pbS ? (bondS ^ 0xfu) : bondS) : nb; c1 = nb; }
-    else { c2 = end ? bondE : next; c1 = end ? bondE : (prev ^ 0xfu); }
+    // Every candidate is a named value, so each choice below is a select and
+    // a wave whose lanes take different cases runs no divergent branch.
+    const uint32_t c2S = pbS ? (bondS ^ 0xfu) : bondS;
+    const uint32_t c2sl = bondS ? c2S : nb;
+    const uint32_t c2no = end ? bondE : next;
+    const uint32_t c1no = end ? bondE : (prev ^ 0xfu);
+    const uint32_t c2 = sl ? c2sl : c2no;
+    const uint32_t c1 = sl ? nb : c1no;
     const uint32_t X2 = P + G::delta(c2);
     // split: X1 = (pbS ? X2 : P) + delta(nPrev), the offset taken from the table word
-    const uint32_t X1 = (sl && bondS) ? (pbS ? X2 : P) + wp : P + G::delta(c1);
+    const uint32_t X1split = (pbS ? X2 : P) + wp;
+    const uint32_t X1other = P + G::delta(c1);
+    const uint32_t X1 = (sl & (bondS != 0u)) ? X1split : X1other;
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     const uint32_t x1 = lat[a1];
     const uint32_t x2 = lat[a2];
-    // The four cases as selects and one guarded write block (a wave whose lanes
-    // take different cases then runs one path instead of all of them).
+    // The four cases as selects and one guarded write block.
     const bool occ = (w & 0xffu) != 0u;
     const bool gate = ((r >> 4) & 1u) == 0u;            // cases without stored length
     // grow the chain end out of the stored length
     const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
     const bool okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
-    const uint32_t pbG = pbS ^ 1u, ins = pbG ? nb : (nb ^ 0xfu), labG = (w >> 9) & 3u;
+    const uint32_t pbG = pbS ^ 1u, labG = (w >> 9) & 3u;
+    const uint32_t ins = pbG ? nb : (nb ^ 0xfu);
     const uint32_t wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
     const uint32_t x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
     // split the chosen bond through an empty site
     const bool okSplit = sl & (bondS != 0u) & !G::out(X2 | X1) & ((x1 & 0xffu) == 0u);
     const uint32_t labS = (w >> (9 + pbS)) & 1u;
     const uint32_t x1S = x1 | nNext | (nPrev << 4) | (labS << 9);
-    const uint32_t wS = pbS ? ((w & ~(0xf0u | kSL | kL1)) | (nNext << 4))
-                            : ((w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9));
-    const uint32_t x2S = pbS ? ((x2 & ~0xfu) | nPrev) : ((x2 & ~0xf0u) | (nNext << 4));
+    const uint32_t wSprev = (w & ~(0xf0u | kSL | kL1)) | (nNext << 4);
+    const uint32_t wSnext = (w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9);
+    const uint32_t wS = pbS ? wSprev : wSnext;
+    const uint32_t x2Sprev = (x2 & ~0xfu) | nPrev, x2Snext = (x2 & ~0xf0u) | (nNext << 4);
+    const uint32_t x2S = pbS ? x2Sprev : x2Snext;
     // retract a chain end onto its neighbour as stored length
     const bool okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
     const uint32_t pbE = (next == 0u) ? 1u : 0u;
     const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
     const uint32_t wE = w & ~(kL0 | (0xfu << (4 * pbE)));
     const uint32_t x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
-    // fold the monomer onto its prev (r bit 0) or next neighbour as stored length
-    uint32_t bond;
-    const bool bondOk = bond_add_valid(next, prev, bond);
+    // fold the monomer onto its prev (r bit 0) or next neighbour as stored
+    // length; bond_add_valid(next, prev) written without an early return
+    const uint32_t orb = next | prev, andb = next & prev;
+    const uint32_t bond = (orb == 0xfu) ? andb : orb;
+    const bool bondOk = ((orb == 0xfu) | (andb == 0u)) & (bond != 0x3u) & (bond != 0xcu);
     const uint32_t pbF = r & 1u, labF = (w >> 9) & 1u;
-    const bool okFold = !sl & gate & !end & !G::out(X1 | X2) & bondOk & ((((pbF ? x1 : x2)) & kSL) == 0u);
-    const uint32_t x1f = pbF ? (((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (labF << 9)) : x1;
-    const uint32_t x2f = pbF ? x2 : (x2 | kSL | (labF << 10));
+    const uint32_t xF = pbF ? x1 : x2;
+    const bool okFold = !sl & gate & !end & !G::out(X1 | X2) & bondOk & ((xF & kSL) == 0u);
+    const uint32_t x1fprev = ((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (labF << 9);
+    const uint32_t x2fnext = x2 | kSL | (labF << 10);
+    const uint32_t x1f = pbF ? x1fprev : x1;
+    const uint32_t x2f = pbF ? x2 : x2fnext;
     const uint32_t x1F = (x1f & ~0xfu) | bond, x2F = (x2f & ~0xf0u) | (bond << 4);
     const uint32_t wF = w & ~(kL0 | 0xffu);
     if (occ & (okGrow | okSplit | okEnd | okFold)) {
