@@ -22,6 +22,21 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
     return c;
 }
 
+// The same generator for a counter that differs between lanes: each round's
+// two three-way XORs as one v_bitop3_b32 each (truth table 0x96) instead of
+// two v_xor_b32.  A tile-uniform counter keeps the plain form, which the
+// compiler runs on the scalar unit.
+__device__ __forceinline__ uint4 philox4x32_10_lanes(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4(__builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c.y, k0, 0x96), (uint32_t)p1,
+                       __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c.w, k1, 0x96), (uint32_t)p0);
+    }
+    return c;
+}
+
 __device__ __forceinline__ uint32_t word_of(const uint4& w, int j) {
     return j == 0 ? w.x : j == 1 ? w.y : j == 2 ? w.z : w.w;
 }
@@ -98,13 +113,14 @@ struct PTile {
     static constexpr int NSITE = PAD ? (1 << (SV + LV)) : TT * TU * TV;
     __device__ static __forceinline__ uint32_t pack(uint32_t t, uint32_t u, uint32_t v) { return t | (u << SU) | (v << SV); }
     // Bond code b -> offset of P: bits 0-2 of b spread to the field origins
-    // 0, SU, SV by one 24-bit multiply (the three shifted copies of b & 7 sit
-    // at bits [0,2], [SU-1,SU+1], [SV-2,SV], disjoint since LT, LU >= 3, so
-    // no carries), then (1,1,1) subtracted when bit 3 is set.
+    // 0, SU, SV by one 24-bit multiply (the three shifted copies of the 4-bit
+    // code sit at bits [0,3], [SU-1,SU+2], [SV-2,SV+1], disjoint since
+    // LT, LU >= 4, so no carries; bit 3 lands outside ONES), then (1,1,1)
+    // subtracted when bit 3 is set.  Callers hold b = site & 15 already.
     static constexpr uint32_t SPREAD = 1u | (1u << (SU - 1)) | (1u << (SV - 2));
-    static_assert(LT >= 3 && LU >= 3, "spread multiply needs tile edges >= 8");
+    static_assert(LT >= 4 && LU >= 4, "spread multiply needs tile edges >= 16 in t and u");
     __device__ static __forceinline__ uint32_t delta(uint32_t b) {
-        const uint32_t d = __umul24(b & 7u, SPREAD) & ONES;
+        const uint32_t d = __umul24(b & 15u, SPREAD) & ONES;
         return d - (ONES & (uint32_t)((int32_t)(b << 28) >> 31));
     }
     __device__ static __forceinline__ bool out(uint32_t P) { return (P & GUARD) != 0u; }

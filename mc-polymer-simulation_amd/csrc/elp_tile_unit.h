@@ -151,7 +151,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
         uint4 sw[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; k++)
-            sw[k] = philox4x32_10(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
+            sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
         const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
         // unrolled, so that word j of each Philox output is a fixed register
 #pragma unroll
