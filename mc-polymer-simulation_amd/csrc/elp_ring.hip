@@ -303,12 +303,16 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
     uint32_t ns = lat[na];
     const uint32_t ds = lat[da];
-    const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u;
+    // the parts of both outcomes that need neither read, computed while the
+    // reads are in flight (the empty statement keeps them ahead of the wait)
+    const uint32_t s1mf = nb1 | ((s >> 1) & 0x10u), s1no = nb1 | (s & 0x70u), d1mf = nb2 | (s & 0x10u);
+    const bool inTile = !G::out(Pn | Pd);
+    asm volatile("" ::"v"(s1mf), "v"(s1no), "v"(d1mf));
     // both outcomes as selects, one guarded write block (no early returns)
-    const bool ok = !G::out(Pn | Pd) & ((sl | nsl) != 0u) & (ds == 0u);
+    const bool ok = inTile & (((s | ns) & 0x40u) != 0u) & (ds == 0u);
     const bool moveFirst = forw_move_first(s, ns, r);
-    const uint32_t s1 = moveFirst ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
-    const uint32_t d1 = nb2 | (moveFirst ? (label & 0x10u) : ((ns & 0x20u) >> 1));
+    const uint32_t s1 = moveFirst ? s1mf : s1no;
+    const uint32_t d1 = moveFirst ? d1mf : (nb2 | ((ns >> 1) & 0x10u));
     if (ok) {
         lat[a] = (uint8_t)s1;
         lat[da] = (uint8_t)d1;
@@ -344,12 +348,19 @@ __device__ __forceinline__ uint32_t back_move_first(uint32_t s, uint32_t ws, uin
     return bit6_mask(~s & (ws | (r << 6)));
 }
 // TransBack's new site and fold-target bytes: the source monomer joins the
-// site (moveFirst) or the fold target.
-__device__ __forceinline__ uint32_t back_site(uint32_t mf, uint32_t s, uint32_t ss, uint32_t newNext) {
-    return newNext | bitsel(mf, ((s & 0x30u) << 1) | (ss & 0x30u) | 0x40u, s & 0x70u);
+// site (moveFirst) or the fold target.  BackParts holds what the site and
+// source bytes fix before the target and the combine code are read.
+struct BackParts {
+    uint32_t siteMf, siteNo, tgtNo;
+};
+__device__ __forceinline__ BackParts back_parts(uint32_t s, uint32_t ss) {
+    return BackParts{((s & 0x30u) << 1) | (ss & 0x30u) | 0x40u, s & 0x70u, ((ss & 0x30u) << 1) | 0x40u};
 }
-__device__ __forceinline__ uint32_t back_target(uint32_t mf, uint32_t ss, uint32_t ws) {
-    return bitsel(mf, ws, (ws & 0x3fu) | ((ss & 0x30u) << 1) | 0x40u);
+__device__ __forceinline__ uint32_t back_site(uint32_t mf, const BackParts& b, uint32_t newNext) {
+    return newNext | bitsel(mf, b.siteMf, b.siteNo);
+}
+__device__ __forceinline__ uint32_t back_target(uint32_t mf, const BackParts& b, uint32_t ws) {
+    return bitsel(mf, ws, (ws & 0x3fu) | b.tgtNo);
 }
 // DiffuseSL's new site and partner bytes, m = bit6_mask(s): the stored-length
 // monomer crosses the bond towards the partner (m set) or onto the site.
@@ -375,12 +386,16 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
     const uint32_t newNext = cc & 0xfu;
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
+    // what the site and source fix, computed while the last reads are in
+    // flight (the empty statement keeps it ahead of the wait)
+    const BackParts bp = back_parts(s, ss);
+    const bool pre = (s != 0u) & !G::out(Ps | Pw);
+    asm volatile("" ::"v"(bp.siteMf), "v"(bp.siteNo), "v"(bp.tgtNo));
     const uint32_t sl = s & 0x40u, wsl = ws & 0x40u;
-    const bool ok = (s != 0u) & !G::out(Ps | Pw) & back_ok_bits(s, ss, ws, cc) &
-                    !((rules != 0u) & ((sl | wsl) != 0u) & ((r & 6u) != 0u));
+    const bool ok = pre & back_ok_bits(s, ss, ws, cc) & !((rules != 0u) & ((sl | wsl) != 0u) & ((r & 6u) != 0u));
     const uint32_t mf = back_move_first(s, ws, r);
-    const uint32_t s1 = back_site(mf, s, ss, newNext);
-    const uint32_t w1 = back_target(mf, ss, ws);
+    const uint32_t s1 = back_site(mf, bp, newNext);
+    const uint32_t w1 = back_target(mf, bp, ws);
     if (ok) {
         lat[sa] = 0;
         lat[a] = (uint8_t)s1;
@@ -452,8 +467,9 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & back_ok_bits(s, ns, x, cc) &
                       !(((sl | xsl) != 0u) & ((r & 6u) != 0u));
     const uint32_t mfTB = back_move_first(s, x, r);
-    const uint32_t sTB = back_site(mfTB, s, ns, newNext);
-    const uint32_t xTB = back_target(mfTB, ns, x);
+    const BackParts bp = back_parts(s, ns);
+    const uint32_t sTB = back_site(mfTB, bp, newNext);
+    const uint32_t xTB = back_target(mfTB, bp, x);
     if (okTF | okDSL | okTB) {
         lat[a] = (uint8_t)(okTF ? sTF : okDSL ? sDSL : sTB);
         if (okDSL | okTB | (okTF & !mfTF)) lat[na] = (uint8_t)(okTF ? (ns & 0x1fu) : okDSL ? nDSL : 0u);
