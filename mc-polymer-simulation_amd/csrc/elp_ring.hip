@@ -364,12 +364,20 @@ __device__ __forceinline__ uint32_t back_target(uint32_t mf, const BackParts& b,
 }
 // DiffuseSL's new site and partner bytes, m = bit6_mask(s): the stored-length
 // monomer crosses the bond towards the partner (m set) or onto the site.
-__device__ __forceinline__ uint32_t diffuse_site(uint32_t m, uint32_t s, uint32_t ns) {
+// DiffuseParts holds what the site byte alone fixes.
+struct DiffuseParts {
+    uint32_t m, siteTo, siteFrom, partnerTo;
+};
+__device__ __forceinline__ DiffuseParts diffuse_parts(uint32_t s) {
     const uint32_t next = s & 0xfu;
-    return bitsel(m, next | ((s >> 1) & 0x10u), next | ((s << 1) & 0x20u) | ((ns >> 1) & 0x10u) | 0x40u);
+    return DiffuseParts{bit6_mask(s), next | ((s >> 1) & 0x10u), next | ((s << 1) & 0x20u) | 0x40u,
+                        ((s & 0x10u) << 1) | 0x40u};
 }
-__device__ __forceinline__ uint32_t diffuse_partner(uint32_t m, uint32_t s, uint32_t ns) {
-    return bitsel(m, ns | ((s & 0x10u) << 1) | 0x40u, ns & 0x1fu);
+__device__ __forceinline__ uint32_t diffuse_site(const DiffuseParts& d, uint32_t ns) {
+    return bitsel(d.m, d.siteTo, d.siteFrom | ((ns >> 1) & 0x10u));
+}
+__device__ __forceinline__ uint32_t diffuse_partner(const DiffuseParts& d, uint32_t ns) {
+    return bitsel(d.m, ns | d.partnerTo, ns & 0x1fu);
 }
 
 template <class G, bool TABLE = false>
@@ -411,12 +419,16 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
     const uint32_t Pn = P + G::delta(next);
     const uint32_t na = G::addr(Pn);
     uint32_t ns = lat[na];
+    // what the site byte fixes, computed while the read is in flight (the
+    // empty statement keeps it ahead of the wait)
+    const DiffuseParts dp = diffuse_parts(s);
+    const bool inTile = !G::out(Pn);
+    asm volatile("" ::"v"(dp.m), "v"(dp.siteTo), "v"(dp.siteFrom), "v"(dp.partnerTo));
     // stored length at exactly one end; an empty site (next = 0) reads itself
     // as partner, so neither end holds it and nothing moves
-    const bool ok = !G::out(Pn) & (((ns ^ s) & 0x40u) != 0u);
-    const uint32_t m = bit6_mask(s);
-    const uint32_t s1 = diffuse_site(m, s, ns);
-    const uint32_t n1 = diffuse_partner(m, s, ns);
+    const bool ok = inTile & (((ns ^ s) & 0x40u) != 0u);
+    const uint32_t s1 = diffuse_site(dp, ns);
+    const uint32_t n1 = diffuse_partner(dp, ns);
     if (ok) {
         lat[a] = (uint8_t)s1;
         lat[na] = (uint8_t)n1;
@@ -459,9 +471,9 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
     const uint32_t xTF = nb2 | (mfTF ? (label & 0x10u) : ((ns & 0x20u) >> 1));
     // DiffuseSL: stored length at exactly one end
     const bool okDSL = isDSL & !G::out(Pn) & (((ns ^ s) & 0x40u) != 0u);
-    const uint32_t mDSL = bit6_mask(s);
-    const uint32_t sDSL = diffuse_site(mDSL, s, ns);
-    const uint32_t nDSL = diffuse_partner(mDSL, s, ns);
+    const DiffuseParts dp = diffuse_parts(s);
+    const uint32_t sDSL = diffuse_site(dp, ns);
+    const uint32_t nDSL = diffuse_partner(dp, ns);
     // TransBack: fold the source onto the site or the third corner; a fold
     // forced by stored length at one end is taken with probability 1/4
     const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & back_ok_bits(s, ns, x, cc) &
