@@ -174,8 +174,12 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
         }
     }
 
-    // LDS -> HBM
-    tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
+    // LDS -> HBM.  The thread index passes through an empty statement, so the
+    // compiler recomputes the box indices here instead of keeping the load's
+    // (16 64-bit addresses) live in registers through the whole loop.
+    int tidStore = tid;
+    asm volatile("" : "+v"(tidStore));
+    tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tidStore);
 }
 
 // Workgroup of a batched launch: global tile g (XCD-aware order over all
