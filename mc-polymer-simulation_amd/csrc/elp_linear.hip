@@ -319,13 +319,26 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     // The four cases as selects and one guarded write block.
     const bool occ = (w & 0xffu) != 0u;
     const bool gate = ((r >> 4) & 1u) == 0u;            // cases without stored length
-    // grow the chain end out of the stored length
-    const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
-    const bool okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
-    const uint32_t pbG = pbS ^ 1u, labG = (w >> 9) & 3u;
-    const uint32_t ins = pbG ? nb : (nb ^ 0xfu);
-    const uint32_t wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
-    const uint32_t x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
+    // Growing and retracting need a chain end (bondS == 0 only at an end), 2
+    // of every N monomers: a wave with no occupied chain-end site skips both
+    // cases on one scalar branch, the other waves compute them as selects.
+    bool okGrow = false, okEnd = false;
+    uint32_t wG = 0u, x1G = 0u, wE = 0u, x1E = 0u;
+    if (__ballot(occ & end) != 0ull) {
+        // grow the chain end out of the stored length
+        const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
+        okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
+        const uint32_t pbG = pbS ^ 1u, labG = (w >> 9) & 3u;
+        const uint32_t ins = pbG ? nb : (nb ^ 0xfu);
+        wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
+        x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
+        // retract a chain end onto its neighbour as stored length
+        okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
+        const uint32_t pbE = (next == 0u) ? 1u : 0u;
+        const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
+        wE = w & ~(kL0 | (0xfu << (4 * pbE)));
+        x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
+    }
     // split the chosen bond through an empty site
     const bool okSplit = sl & (bondS != 0u) & !G::out(X2 | X1) & ((x1 & 0xffu) == 0u);
     const uint32_t labS = (w >> (9 + pbS)) & 1u;
@@ -335,12 +348,6 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     const uint32_t wS = pbS ? wSprev : wSnext;
     const uint32_t x2Sprev = (x2 & ~0xfu) | nPrev, x2Snext = (x2 & ~0xf0u) | (nNext << 4);
     const uint32_t x2S = pbS ? x2Sprev : x2Snext;
-    // retract a chain end onto its neighbour as stored length
-    const bool okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
-    const uint32_t pbE = (next == 0u) ? 1u : 0u;
-    const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
-    const uint32_t wE = w & ~(kL0 | (0xfu << (4 * pbE)));
-    const uint32_t x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
     // fold the monomer onto its prev (r bit 0) or next neighbour as stored
     // length; bond_add_valid(next, prev) written without an early return
     const uint32_t orb = next | prev, andb = next & prev;
