@@ -285,9 +285,7 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
 // holds fold_pair in bits 24-31 above the packed offset of its first bond
 // (nPrev) in bits 0-23, so the split's X1 is one add of that word.
 template <class G>
-__device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
-    const uint32_t a = G::addr(P);
-    uint32_t w = lat[a];
+__device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     // stored-length cases: split the chosen bond, or grow the end
@@ -319,26 +317,19 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
     // The four cases as selects and one guarded write block.
     const bool occ = (w & 0xffu) != 0u;
     const bool gate = ((r >> 4) & 1u) == 0u;            // cases without stored length
-    // Growing and retracting need a chain end (bondS == 0 only at an end), 2
-    // of every N monomers: a wave with no occupied chain-end site skips both
-    // cases on one scalar branch, the other waves compute them as selects.
-    bool okGrow = false, okEnd = false;
-    uint32_t wG = 0u, x1G = 0u, wE = 0u, x1E = 0u;
-    if (__ballot(occ & end) != 0ull) {
-        // grow the chain end out of the stored length
-        const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
-        okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
-        const uint32_t pbG = pbS ^ 1u, labG = (w >> 9) & 3u;
-        const uint32_t ins = pbG ? nb : (nb ^ 0xfu);
-        wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
-        x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
-        // retract a chain end onto its neighbour as stored length
-        okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
-        const uint32_t pbE = (next == 0u) ? 1u : 0u;
-        const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
-        wE = w & ~(kL0 | (0xfu << (4 * pbE)));
-        x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
-    }
+    // grow the chain end out of the stored length
+    const bool nbBad = (nb == 0u) | (nb == 3u) | (nb == 0xcu) | (nb == 0xfu);
+    const bool okGrow = sl & (bondS == 0u) & !nbBad & !G::out(X1) & ((x1 & 0xffu) == 0u);
+    const uint32_t pbG = pbS ^ 1u, labG = (w >> 9) & 3u;
+    const uint32_t ins = pbG ? nb : (nb ^ 0xfu);
+    const uint32_t wG = (w & ~(kL0 | kL1 | kSL)) | (((labG >> pbG) & 1u) << 9) | (ins << (4 * (pbG ^ 1u)));
+    const uint32_t x1G = x1 | (((labG >> (pbG ^ 1u)) & 1u) << 9) | (ins << (4 * pbG));
+    // retract a chain end onto its neighbour as stored length
+    const bool okEnd = !sl & gate & end & (((r >> 5) & 7u) == 0u) & !G::out(X1) & ((x1 & kSL) == 0u);
+    const uint32_t pbE = (next == 0u) ? 1u : 0u;
+    const uint32_t nl = (((w >> 9) & 1u) << (pbE ^ 1u)) | (((x1 >> 9) & 1u) << pbE);
+    const uint32_t wE = w & ~(kL0 | (0xfu << (4 * pbE)));
+    const uint32_t x1E = (x1 & ~(kL0 | kL1 | (0xfu << (4 * (pbE ^ 1u))))) | (nl << 9) | kSL;
     // split the chosen bond through an empty site
     const bool okSplit = sl & (bondS != 0u) & !G::out(X2 | X1) & ((x1 & 0xffu) == 0u);
     const uint32_t labS = (w >> (9 + pbS)) & 1u;
@@ -367,6 +358,72 @@ __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint3
         lat[a1] = (uint16_t)(okGrow ? x1G : okSplit ? x1S : okEnd ? x1E : x1F);
         if (okSplit | okFold) lat[a2] = (uint16_t)(okSplit ? x2S : x2F);
     }
+}
+
+// TransMove of a wave in which no occupied site is a chain end: every
+// occupied lane has next != 0 and prev != 0, so only the split (stored
+// length) and the fold (none) can happen, the split's bond is never 0 and
+// the fold needs no end test.  Same outcome as lin_trans_move_ends for such a
+// wave; empty lanes read in-tile sites and write nothing.
+template <class G>
+__device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a,
+                                                        uint32_t w) {
+    const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
+    const bool sl = (w & kSL) != 0u;
+    const uint32_t pbS = (r >> 2) & 1u;
+    const uint32_t bondS = pbS ? prev : next;
+    const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
+    const uint32_t pair = wp >> 24;
+    const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
+    // split: X2 = bond partner, X1 = inserted site; fold: X2 = Pn, X1 = Pp
+    const uint32_t c2 = sl ? (pbS ? (bondS ^ 0xfu) : bondS) : next;
+    const uint32_t X2 = P + G::delta(c2);
+    const uint32_t X1 = sl ? (pbS ? X2 : P) + wp : P + G::delta(prev ^ 0xfu);
+    const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
+    const uint32_t x1 = lat[a1];
+    const uint32_t x2 = lat[a2];
+    const bool occ = (w & 0xffu) != 0u;
+    const bool gate = ((r >> 4) & 1u) == 0u;
+    const bool inTile = !G::out(X2 | X1);
+    // split the chosen bond through an empty site
+    const bool okSplit = sl & inTile & ((x1 & 0xffu) == 0u);
+    const uint32_t labS = (w >> (9 + pbS)) & 1u;
+    const uint32_t x1S = x1 | nNext | (nPrev << 4) | (labS << 9);
+    const uint32_t wSprev = (w & ~(0xf0u | kSL | kL1)) | (nNext << 4);
+    const uint32_t wSnext = (w & ~(0xfu | kSL | kL0 | kL1)) | nPrev | (((w >> 10) & 1u) << 9);
+    const uint32_t wS = pbS ? wSprev : wSnext;
+    const uint32_t x2Sprev = (x2 & ~0xfu) | nPrev, x2Snext = (x2 & ~0xf0u) | (nNext << 4);
+    const uint32_t x2S = pbS ? x2Sprev : x2Snext;
+    // fold the monomer onto its prev (r bit 0) or next neighbour
+    const uint32_t orb = next | prev, andb = next & prev;
+    const uint32_t bond = (orb == 0xfu) ? andb : orb;
+    const bool bondOk = ((orb == 0xfu) | (andb == 0u)) & (bond != 0x3u) & (bond != 0xcu);
+    const uint32_t pbF = r & 1u, labF = (w >> 9) & 1u;
+    const uint32_t xF = pbF ? x1 : x2;
+    const bool okFold = !sl & gate & inTile & bondOk & ((xF & kSL) == 0u);
+    const uint32_t x1fprev = ((x1 | kSL) & ~kL0) | (((x1 >> 9) & 1u) << 10) | (labF << 9);
+    const uint32_t x2fnext = x2 | kSL | (labF << 10);
+    const uint32_t x1f = pbF ? x1fprev : x1;
+    const uint32_t x2f = pbF ? x2 : x2fnext;
+    const uint32_t x1F = (x1f & ~0xfu) | bond, x2F = (x2f & ~0xf0u) | (bond << 4);
+    const uint32_t wF = w & ~(kL0 | 0xffu);
+    // okSplit implies sl and okFold !sl, so sl picks the outcome
+    if (occ & (okSplit | okFold)) {
+        lat[a] = (uint16_t)(sl ? wS : wF);
+        lat[a1] = (uint16_t)(sl ? x1S : x1F);
+        lat[a2] = (uint16_t)(sl ? x2S : x2F);
+    }
+}
+
+// Growing and retracting need a chain end, 2 of every N monomers: a wave with
+// no occupied chain-end site takes the two-case form on one scalar branch.
+template <class G>
+__device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
+    const uint32_t a = G::addr(P);
+    const uint32_t w = lat[a];
+    const bool occEnd = ((w & 0xffu) != 0u) & (((w & 0xfu) == 0u) | ((w & 0xf0u) == 0u));
+    if (__ballot(occEnd) != 0ull) lin_trans_move_ends<G>(lat, P, r, a, w);
+    else lin_trans_move_interior<G>(lat, P, r, a, w);
 }
 
 // linpol's substep sequence {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490)
