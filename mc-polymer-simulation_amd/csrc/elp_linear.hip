@@ -285,14 +285,14 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
 // holds fold_pair in bits 24-31 above the packed offset of its first bond
 // (nPrev) in bits 0-23, so the split's X1 is one add of that word.
 template <class G>
-__device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w) {
+__device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w,
+                                                    uint32_t wp) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     // stored-length cases: split the chosen bond, or grow the end
     const uint32_t pbS = (r >> 2) & 1u;
     const uint32_t bondS = pbS ? prev : next;
     const uint32_t nb = (r >> 3) & 0xfu;
-    const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
     const uint32_t pair = wp >> 24;
     const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
     // cases without stored length: retract an end, or fold an interior monomer
@@ -367,12 +367,11 @@ __device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, u
 // wave; empty lanes read in-tile sites and write nothing.
 template <class G>
 __device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a,
-                                                        uint32_t w) {
+                                                        uint32_t w, uint32_t wp) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     const uint32_t pbS = (r >> 2) & 1u;
     const uint32_t bondS = pbS ? prev : next;
-    const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
     const uint32_t pair = wp >> 24;
     const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
     // split: X2 = bond partner, X1 = inserted site; fold: X2 = Pn, X1 = Pp
@@ -417,13 +416,17 @@ __device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t 
 
 // Growing and retracting need a chain end, 2 of every N monomers: a wave with
 // no occupied chain-end site takes the two-case form on one scalar branch.
+// Both forms read the table word of the chosen bond (bondS, r & 3), which is
+// read before the branch so that the read and the test overlap.
 template <class G>
 __device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     const uint32_t w = lat[a];
+    const uint32_t bondS = ((r >> 2) & 1u) ? (w >> 4) & 0xfu : w & 0xfu;
+    const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
     const bool occEnd = ((w & 0xffu) != 0u) & (((w & 0xfu) == 0u) | ((w & 0xf0u) == 0u));
-    if (__ballot(occEnd) != 0ull) lin_trans_move_ends<G>(lat, P, r, a, w);
-    else lin_trans_move_interior<G>(lat, P, r, a, w);
+    if (__ballot(occEnd) != 0ull) lin_trans_move_ends<G>(lat, P, r, a, w, wp);
+    else lin_trans_move_interior<G>(lat, P, r, a, w, wp);
 }
 
 // linpol's substep sequence {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490)
