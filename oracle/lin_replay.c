@@ -21,7 +21,12 @@
  *
  * Pinned by: Philox known-answer vectors; slot-order independence; chain
  * invariants (monomer count, one start and one end per chain, label
- * sequence of every chain unchanged, host recovery of every id).
+ * sequence of every chain unchanged, host recovery of every id); the exact
+ * forward/reverse rate ratio of every move, 2^(change in stored length), by
+ * enumeration of every random byte at every site (orc_lin_move,
+ * tests/test_lin_balance.py, which also shows that each planted wrong gate
+ * breaks it); and the stationary law of one short chain, weight 2^(stored
+ * length) over its reachable conformations (tests/test_lin_stationary_gpu.py).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -72,17 +77,18 @@ static int uvalid_add(uint32_t a, uint32_t b, uint32_t* c) {
 }
 static int unit_ok(uint32_t u) { return !(u == 0x0 || u == 0x3 || u == 0xc || u == 0xf); }
 
-/* per-move random bits: the slot's fixed byte (engine mode) or a draw from
-   the lane's LFSR113 state where the reference kernel calls Rng4
-   (gpupol/linpol.cu:48-59, :123) */
-typedef struct { int ref; uint32_t fixed; uint32_t* st; } LDraw;
-static uint32_t lfsr113_dev(uint32_t* s) {
-    uint32_t b;
-    b = ((s[0] << 6) ^ s[0]) >> 13;  s[0] = ((s[0] & 4294967294u) << 18) ^ b;
-    b = ((s[1] << 2) ^ s[1]) >> 27;  s[1] = ((s[1] & 4294967288u) << 2) ^ b;
-    b = ((s[2] << 13) ^ s[2]) >> 21; s[2] = ((s[2] & 4294967280u) << 7) ^ b;
-    b = ((s[3] << 3) ^ s[3]) >> 12;  s[3] = ((s[3] & 4294967168u) << 13) ^ b;
-    return s[0] ^ s[1] ^ s[2] ^ s[3];
+/* per-move random bits: the slot's fixed byte (where the reference kernel
+   draws from its per-lane LFSR113 state, gpupol/linpol.cu:123, the engine
+   reads this byte) */
+typedef struct { uint32_t fixed; } LDraw;
+
+/* Test-only planted mistakes (orc_lin_plant), to show that the rate checks
+   can tell a wrong gate; 0 = the rules as written. */
+static int g_plant = 0;
+int orc_lin_plant(int which) {
+    if (which < 0 || which > 3) return -1;
+    g_plant = which;
+    return 0;
 }
 
 static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
@@ -97,7 +103,7 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
     next = nextPrev & 0xf;
     slabSite = slab[site / 8];
     if (!nextPrev) return;
-    uint32_t rnd = rd->ref ? lfsr113_dev(rd->st) : rd->fixed;
+    uint32_t rnd = rd->fixed;
     if ((slabSite >> (16 | (site & 0x7))) & 0x1) {
         prevBond = (rnd & 0x4) >> 2;
         bond = (nextPrev >> (prevBond * 4)) & 0xf;
@@ -129,8 +135,9 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
         nPrev = (X->tr[bond / 4] >> (4 * (2 * (bond % 4) + (rnd & 0x1)))) & 0xf;
         nNext = uadd((~nPrev) & 0xf, bond);
         temp = nPrev;
-        nPrev = (rnd & 0x2) ? nPrev : nNext;
-        nNext = (rnd & 0x2) ? nNext : temp;
+        uint32_t order = (g_plant == 3) ? 0x2 : (rnd & 0x2);
+        nPrev = order ? nPrev : nNext;
+        nNext = order ? nNext : temp;
         bondSite = add_unit(X, bond ^ (0xf * prevBond), site);
         oldNextSite = prevBond ? site : bondSite;
         oldPrevSite = prevBond ? bondSite : site;
@@ -161,9 +168,10 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
         slab[site / 8] = slabSite;
         slab[newSite / 8] |= label << ((newSite & 0x7) * 2);
     } else {
-        if (((rnd >> 4) & 0x1) != 0) return;
-        if (next == 0 || prev == 0) {
-            if (((rnd >> 5) & 0x7) != 0) return;
+        int end = (next == 0 || prev == 0);
+        if (((rnd >> 4) & 0x1) != 0 && !(g_plant == 2 && !end)) return;
+        if (end) {
+            if (((rnd >> 5) & (g_plant == 1 ? 0x3 : 0x7)) != 0) return;
             if (next == 0) { bond = prev ^ 0xf; prevBond = 1; }
             else { bond = next; prevBond = 0; }
             if (edge_outside(X, site, bond)) return;
@@ -336,7 +344,7 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
                                 uint32_t site = lidx(&X, t, u, v);
                                 if (s == 0) diffuse_sl(&X, site);
                                 else {
-                                    LDraw rd = {0, (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff, NULL};
+                                    LDraw rd = {(philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff};
                                     trans_move(&X, site, &rd);
                                 }
                             }
@@ -362,69 +370,60 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
 }
 
 /*
- * Reference mode: the schedule of gpupol/linpol.cu itself -- 32^3 tiles at a
- * fresh offset per time unit (origins), one lane per 4x4x4 cell (lane bits
- * t:0-2 u:3-5 v:6-8, :440-444), per-lane LFSR113 states rngp/rngl from and
- * back to the seed table (gpupol/gpupol_init.c:23-34); per iteration one rngl
- * draw masked by 0x88f selects the DiffuseSL site and a second one the
- * TransMove site (:483-490).  Checked against the reference kernel compiled
- * unmodified (oracle/refkernel_lin.hip).
+ * One move on a T x U x V buffer of site words (t fastest; T, U, V multiples
+ * of 4), treated as one tile with no wrap: type 0 DiffuseSL, 1 TransMove with
+ * the random byte r.  For enumerating transition probabilities of the move
+ * set.  Returns 0, or -1 on bad arguments.
  */
-int orc_lin_run_ref(uint16_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {
-    if (L % 32) return -1;
-    const int nw = L / 32, W = 32;
+int orc_lin_move(uint16_t* lat, int T, int U, int V, int type, int t, int u, int v, uint32_t r) {
+    if (T % 4 || U % 4 || V % 4 || T <= 0 || U <= 0 || V <= 0) return -1;
+    if (t < 0 || t >= T || u < 0 || u >= U || v < 0 || v >= V || type < 0 || type > 1) return -1;
     LTile X;
-    X.T = X.U = X.V = W; X.nct = W / 4; X.ncu = W / 4;
-    int nsite = W * W * W;
-    X.lattice = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 4));
-    X.slab = (uint32_t*)malloc(sizeof(uint32_t) * (nsite / 8));
+    X.T = T; X.U = U; X.V = V; X.nct = T / 4; X.ncu = U / 4;
+    int nsite = T * U * V;
+    X.lattice = (uint32_t*)calloc((size_t)nsite / 4, sizeof(uint32_t));
+    X.slab = (uint32_t*)calloc((size_t)nsite / 8, sizeof(uint32_t));
     gpu_trans(X.tr);
-    for (int i = 0; i < n_time; i++) {
-        int ot = origins[3 * i], ou = origins[3 * i + 1], ov = origins[3 * i + 2];
-        for (int wid = 0; wid < nw * nw * nw; wid++) {
-            int wt = wid % nw, wu = (wid / nw) % nw, wv = wid / (nw * nw);
-            memset(X.lattice, 0, sizeof(uint32_t) * (nsite / 4));
-            memset(X.slab, 0, sizeof(uint32_t) * (nsite / 8));
-            for (int v = 0; v < W; v++)
-                for (int u = 0; u < W; u++)
-                    for (int t = 0; t < W; t++) {
-                        size_t g = (size_t)((ot + wt * W + t) % L) + (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L));
-                        uint32_t w = lat[g], sidx = lidx(&X, t, u, v);
-                        X.lattice[sidx / 4] |= (w & 0xff) << (8 * (sidx % 4));
-                        X.slab[sidx / 8] |= ((w >> 8) & 1) << (16 | (sidx & 7));
-                        X.slab[sidx / 8] |= ((w >> 9) & 3) << ((sidx & 7) * 2);
-                    }
-            uint32_t* st = seeds + (size_t)wid * 512 * 8;
-            for (int it = 0; it < n_step; it++) {
-                for (int s = 0; s < 2; s++) {
-                    uint32_t off = 0;
-                    for (int lid = 0; lid < 512; lid++) off = lfsr113_dev(st + 8 * lid + 4) & 0x88f;  /* identical per lane */
-                    int ot2 = (int)(off & 1) + 2 * (int)((off >> 3) & 1);
-                    int ou2 = (int)((off >> 1) & 1) + 2 * (int)((off >> 7) & 1);
-                    int ov2 = (int)((off >> 2) & 1) + 2 * (int)((off >> 11) & 1);
-                    for (int lid = 0; lid < 512; lid++) {
-                        uint32_t site = lidx(&X, 4 * (lid & 7) + ot2, 4 * ((lid >> 3) & 7) + ou2, 4 * ((lid >> 6) & 7) + ov2);
-                        if (s == 0) diffuse_sl(&X, site);
-                        else {
-                            LDraw rd = {1, 0, st + 8 * lid};
-                            trans_move(&X, site, &rd);
-                        }
-                    }
-                }
+    for (int w = 0; w < V; w++)
+        for (int q = 0; q < U; q++)
+            for (int p = 0; p < T; p++) {
+                uint32_t word = lat[p + T * (q + U * w)], s = lidx(&X, p, q, w);
+                X.lattice[s / 4] |= (word & 0xff) << (8 * (s % 4));
+                X.slab[s / 8] |= ((word >> 8) & 1) << (16 | (s & 7));
+                X.slab[s / 8] |= ((word >> 9) & 3) << ((s & 7) * 2);
             }
-            for (int v = 0; v < W; v++)
-                for (int u = 0; u < W; u++)
-                    for (int t = 0; t < W; t++) {
-                        size_t g = (size_t)((ot + wt * W + t) % L) + (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L));
-                        uint32_t sidx = lidx(&X, t, u, v);
-                        uint32_t w = (X.lattice[sidx / 4] >> (8 * (sidx % 4))) & 0xff;
-                        w |= ((X.slab[sidx / 8] >> (16 | (sidx & 7))) & 1) << 8;
-                        w |= ((X.slab[sidx / 8] >> ((sidx & 7) * 2)) & 3) << 9;
-                        lat[g] = (uint16_t)w;
-                    }
-        }
+    uint32_t site = lidx(&X, t, u, v);
+    if (type == 0) diffuse_sl(&X, site);
+    else {
+        LDraw rd = {r & 0xff};
+        trans_move(&X, site, &rd);
     }
+    for (int w = 0; w < V; w++)
+        for (int q = 0; q < U; q++)
+            for (int p = 0; p < T; p++) {
+                uint32_t s = lidx(&X, p, q, w);
+                uint32_t word = (X.lattice[s / 4] >> (8 * (s % 4))) & 0xff;
+                word |= ((X.slab[s / 8] >> (16 | (s & 7))) & 1) << 8;
+                word |= ((X.slab[s / 8] >> ((s & 7) * 2)) & 3) << 9;
+                lat[p + T * (q + U * w)] = (uint16_t)word;
+            }
     free(X.lattice);
     free(X.slab);
+    return 0;
+}
+
+/*
+ * Every move of one site: out[0] the buffer after DiffuseSL, out[1 + r] after
+ * TransMove with the random byte r (r = 0..255); each out[k] is a full copy
+ * of the T x U x V buffer.  For enumerating transition probabilities.
+ */
+int orc_lin_moves(const uint16_t* lat, int T, int U, int V, int t, int u, int v, uint16_t* out) {
+    size_t n = (size_t)T * U * V;
+    for (int k = 0; k < 257; k++) {
+        uint16_t* o = out + (size_t)k * n;
+        memcpy(o, lat, n * sizeof(uint16_t));
+        int rc = orc_lin_move(o, T, U, V, k ? 1 : 0, t, u, v, k ? (uint32_t)(k - 1) : 0u);
+        if (rc) return rc;
+    }
     return 0;
 }

@@ -23,9 +23,9 @@
  * lattices -- a brute-force check that slots of one substep commute);
  * invariants after every run (monomer count, ring closure, <= 1 stored
  * length per site, cyclic label sequence of every ring unchanged); exact
- * stationary-distribution checks on tiny systems; and, in reference mode,
- * bit-equality with the reference kernel gpupol2/ringpol3.cu compiled
- * unmodified (tests/test_refkernel_gpu.py).
+ * stationary-distribution checks on tiny systems (tests/test_ring_balance.py:
+ * every move has a reverse and the exact forward/reverse rate ratios;
+ * tests/test_learned_gpu.py: the six-monomer ring's exact measure).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -77,22 +77,11 @@ static void gpu_trans(uint32_t tr[4]) {
 }
 
 /* ---------------- per-move random bits ----------------
- * Engine mode: the slot's byte for this substep, fixed in advance.
- * Reference mode: the lane's xorshift128 state, drawn only where the
- * reference kernel calls Rng4 (gpupol2/ringpol3.cu:26-35, :108, :171). */
-typedef struct { int ref; uint32_t fixed; uint32_t* xs; } Draw;
-
-static uint32_t xorshift128(uint32_t* s) {      /* s = (x, y, z, w) of a uint4 */
-    uint32_t t = s[3];
-    t ^= t << 11;
-    t ^= t >> 8;
-    s[3] = s[2]; s[2] = s[1]; s[1] = s[0];
-    t ^= s[0];
-    t ^= s[0] >> 19;
-    s[0] = t;
-    return t;
-}
-static uint32_t draw(Draw* d) { return d->ref ? xorshift128(d->xs) : d->fixed; }
+ * The slot's byte for this substep, fixed in advance (where the reference
+ * kernel draws from its per-lane xorshift128 state, gpupol2/ringpol3.cu:108,
+ * :171, the engine reads this byte). */
+typedef struct { uint32_t fixed; } Draw;
+static uint32_t draw(Draw* d) { return d->fixed; }
 
 /* ---------------- moves on a tile buffer ---------------- */
 typedef struct { uint8_t* b; int T, U, V; uint32_t tr[4]; int rules; } Tile;   /* rules: 0 reference, 1 balanced */
@@ -198,7 +187,7 @@ int orc_ring_move(uint8_t* tile, int T, int U, int V, int type, int t, int u, in
     if (t < 0 || t >= T || u < 0 || u >= U || v < 0 || v >= V) return -1;
     Tile X = {tile, T, U, V, {0, 0, 0, 0}, rules};
     gpu_trans(X.tr);
-    Draw d = {0, r, NULL};
+    Draw d = {r};
     if (type == 0) trans_forw(&X, t, u, v, &d);
     else if (type == 1) diffuse_sl(&X, t, u, v);
     else if (type == 2) trans_back(&X, t, u, v, &d);
@@ -270,7 +259,7 @@ int orc_ring_run_rules(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int
                             for (int kk = 0; kk < nslot; kk++) {
                                 int slot = perm[kk], t, u, v;
                                 slot_site(sublat, TT, TU, slot, c, &t, &u, &v);
-                                Draw rd = {0, (philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> (8 * s)) & 0xff, NULL};
+                                Draw rd = {(philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> (8 * s)) & 0xff};
                                 /* move of this slot: the substep's (TF, DSL, TF, TB), or under
                                    balanced rates bits 6-7 of the byte: 0,1 TF, 2 DSL, 3 TB */
                                 int mv = s == 1 ? 1 : (s == 3 ? 2 : 0);
@@ -295,65 +284,8 @@ int orc_ring_run_rules(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int
     return 0;
 }
 
-/*
- * Reference mode: the schedule of gpupol2/ringpol3.cu itself -- 32^3 tiles,
- * one lane per 4x4x4 cell (lane -> cell permutation of :272), per-lane
- * xorshift128 states rngp/rngl taken from and written back to the seed table
- * of gpupol2/gpupol_init.c:22-33 (uint4 pair per lane: rngp, rngl), one rngl
- * draw per iteration giving the four 6-bit cell offsets (:279-291).
- * origins[3i..3i+2] is the accumulated host shift of time unit i modulo L:
- * work-group w (and its seed streams) owns the tile starting at
- * origin + 32*(w%nw, (w/nw)%nw, w/nw^2) (gpupol2/oclpol.c:18-47), so the
- * shift is needed mod L, not mod 32.  Checked against the reference kernel compiled
- * unmodified (oracle/refkernel_ring.hip).
- */
+/* orc_ring_run_rules with the reference's move rates */
 int orc_ring_run(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, int sublat, int n_iter,
                  int n_time, const int32_t* origins, uint64_t tau0, uint32_t seed, uint32_t stream, int order) {
     return orc_ring_run_rules(lat, LT, LU, LV, TT, TU, TV, sublat, n_iter, n_time, origins, tau0, seed, stream, order, 0);
-}
-
-int orc_ring_run_ref(uint8_t* lat, int L, uint32_t* seeds, int n_step, int n_time, const int32_t* origins) {
-    if (L % 32) return -1;
-    const int nw = L / 32, W = 32;
-    Tile X;
-    X.T = X.U = X.V = W;
-    X.rules = 0;
-    X.b = (uint8_t*)malloc((size_t)W * W * W);
-    gpu_trans(X.tr);
-    for (int i = 0; i < n_time; i++) {
-        int ot = origins[3 * i], ou = origins[3 * i + 1], ov = origins[3 * i + 2];
-        for (int wid = 0; wid < nw * nw * nw; wid++) {
-            int wt = wid % nw, wu = (wid / nw) % nw, wv = wid / (nw * nw);
-            for (int v = 0; v < W; v++)
-                for (int u = 0; u < W; u++)
-                    for (int t = 0; t < W; t++)
-                        X.b[t + W * (u + W * v)] = lat[(size_t)((ot + wt * W + t) % L) +
-                                                       (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L))];
-            uint32_t* st = seeds + (size_t)wid * 512 * 8;     /* lane lid: rngp = st[8*lid..+3], rngl = st[8*lid+4..+7] */
-            for (int it = 0; it < n_step; it++) {
-                uint32_t loc = 0;
-                for (int lid = 0; lid < 512; lid++) loc = xorshift128(st + 8 * lid + 4);   /* identical per lane */
-                for (int s = 0; s < 4; s++) {
-                    uint32_t off = (loc >> (6 * s)) & 0x3f;
-                    for (int lid = 0; lid < 512; lid++) {
-                        int cell = ((lid & 0x1f) << 2) | ((lid & 0x60) >> 5) | (lid & 0x180);
-                        int t = 4 * (cell & 7) + (int)(off & 3);
-                        int u = 4 * ((cell >> 3) & 7) + (int)((off >> 2) & 3);
-                        int v = 4 * ((cell >> 6) & 7) + (int)((off >> 4) & 3);
-                        Draw rd = {1, 0, st + 8 * lid};
-                        if (s == 1) diffuse_sl(&X, t, u, v);
-                        else if (s == 3) trans_back(&X, t, u, v, &rd);
-                        else trans_forw(&X, t, u, v, &rd);
-                    }
-                }
-            }
-            for (int v = 0; v < W; v++)
-                for (int u = 0; u < W; u++)
-                    for (int t = 0; t < W; t++)
-                        lat[(size_t)((ot + wt * W + t) % L) + (size_t)L * ((size_t)((ou + wu * W + u) % L) + (size_t)L * ((ov + wv * W + v) % L))] =
-                            X.b[t + W * (u + W * v)];
-        }
-    }
-    free(X.b);
-    return 0;
 }
