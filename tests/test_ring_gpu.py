@@ -81,6 +81,32 @@ def test_ring_bit_exact_L64_noncubic_box():
 
 @pytest.mark.parametrize("tile,sublat,n_iter", [((32, 32, 32), elp.SUBLAT_CUBIC, 51),
                                                 ((16, 16, 16), elp.SUBLAT_DENSE, 13)])
+def test_ring_L96_seeds_0_to_7_bit_exact(tile, sublat, n_iter):
+    """Ring melt L=96, N=1000, density 1.2 (1062 rings), seeds 0..7 from the
+    reference start: four time units (split 1 + 3 across two runs) bit-exact
+    against the C replay, on the drop-in default schedule and the dense one."""
+    for seed in range(8):
+        _say(f"[ring parity] L=96 N=1000 seed={seed} tile={tile} sublat={sublat}")
+        h = elp.RingHost(N=1000, L=96, density=1.2, seed=seed)
+        assert h.n_pol == 1062
+        lat0 = h.lattice()
+        h.load_shift(tile)
+        org = h.origins(tile, 4)
+        eng = elp.Engine("ring", h.L, tile, sublat, n_iter)
+        eng.upload(lat0)
+        eng.run(org[:1], tau0=0, seed=h.seed)
+        eng.run(org[1:], tau0=1, seed=h.seed)
+        got = eng.download()
+        eng.close()
+        want = replay_ring(lat0, h.L, tile, sublat, n_iter, org, 0, h.seed)
+        assert np.count_nonzero(want != lat0) > lat0.size // 8
+        assert np.array_equal(got, want), f"seed {seed}: {np.count_nonzero(got != want)} sites differ"
+        assert site_invariants(got, h.L) == 1062 * 1000
+        h.close()
+
+
+@pytest.mark.parametrize("tile,sublat,n_iter", [((32, 32, 32), elp.SUBLAT_CUBIC, 51),
+                                                ((16, 16, 16), elp.SUBLAT_DENSE, 13)])
 def test_ring_full_size_invariants(tile, sublat, n_iter):
     """Box L=96, N=1000 (1062 rings): after 200 time units every ring is
     intact, the monomer count and every ring's cyclic label sequence are
