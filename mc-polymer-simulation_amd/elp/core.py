@@ -448,7 +448,9 @@ class _Host:
     def __init__(self, N=None, L=None, density=1.2, seed=0, path=None, box_unit=0, init="reference",
                  site_fraction=0.0):
         """box_unit: box edges are truncated to multiples of it (0 = 32, the
-        reference's rule; 16 allows boxes such as 48 for 16^3-tile engines).
+        reference's rule; 16 allows boxes such as 48 for 16^3-tile engines);
+        with a path, the edges of the file's box must be multiples of it (0 =
+        the largest of 32, 16, 8, 4 that divides them).
         init: "reference" (the reference's compact chains) or, for linear
         chains, "saw" (random self-avoiding walks covering site_fraction of
         the box; 0 takes the library default)."""
@@ -478,7 +480,13 @@ class _Host:
         if init not in ("reference", "saw"):
             raise ValueError(f"unknown init {init!r}")
         if path is not None:
-            h = getattr(lib, p + "from_file")(os.fsencode(path), int(seed))
+            if box_unit:
+                f = getattr(lib, p + "from_file_unit")
+                f.restype = C.c_void_p
+                f.argtypes = [C.c_char_p, C.c_uint32, C.c_int]
+                h = f(os.fsencode(path), int(seed), int(box_unit))
+            else:
+                h = getattr(lib, p + "from_file")(os.fsencode(path), int(seed))
         elif init == "saw":
             if not hasattr(lib, p + "new_saw"):
                 raise ElpError("the self-avoiding-walk start exists for linear chains only")

@@ -17,7 +17,10 @@ typedef unsigned int uint;
 /* 2^-32: scales a 32-bit draw to [0,1) (gpupol2/gpupol.h:15, util/rng.h:27) */
 #define ELP_RNG_FAC 2.3283064365386962890625E-10
 
+#ifndef ELP_UINT4_T_DEF
+#define ELP_UINT4_T_DEF
 typedef struct uint4_t { uint x, y, z, w; } uint4_t;
+#endif
 
 /* L'Ecuyer LFSR113 (gpupol2/gpupol.h:84-95, identical to util/rng.h:10-21). */
 static inline uint elp_lfsr113(uint4_t* s) {

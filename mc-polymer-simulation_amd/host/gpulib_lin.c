@@ -13,6 +13,7 @@
 #include <string.h>
 #include "../../include/elp_gpu.h"
 #include "gpupol_lin.h"
+#include "elp_host_common.h"
 
 #define ELPCHK(call)                                                                                  \
     do {                                                                                              \
@@ -135,8 +136,7 @@ int WriteLatticeFileDevice(SimProperties* sp, SimState* ss, GPUDeviceState* dev,
 }
 
 /* ---------------- handle API (bench.py, tests, other drivers) ---------------- */
-typedef struct elp_lin_host { SimProperties sp; SimState ss; } elp_lin_host;
-void elp_lin_host_free(elp_lin_host* h);
+struct elp_lin_host { SimProperties sp; SimState ss; };
 
 /* Fresh start with box edges truncated to multiples of box_unit (0: 32, as the
    reference; a smaller unit, e.g. 16, allows boxes such as 48 for engines with
@@ -146,10 +146,10 @@ elp_lin_host* elp_lin_host_new_unit(int N, int LT, int LU, int LV, double densit
     elp_lin_host* h = (elp_lin_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.polLength = N; h->sp.density = density; h->sp.seed = seed; h->sp.nDevices = 1;
-    h->sp.boxUnit = box_unit;
-    SetBoxDimension(&h->sp, LT, LU, LV);
+    SetBoxDimensionUnit(&h->sp, LT, LU, LV, box_unit ? box_unit : ELP_LIN_TILE);
+    if (!h->sp.L) { elp_lin_host_free(h); return NULL; }
     SimStateInit(&h->ss, &h->sp);
-    GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
+    if (GetBestPolymerPartitioning(&h->sp, &h->ss, N, density)) { elp_lin_host_free(h); return NULL; }
     RedistribSL(&h->ss, &h->sp);
     if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { elp_lin_host_free(h); return NULL; }
     return h;
@@ -168,23 +168,44 @@ elp_lin_host* elp_lin_host_new_saw(int N, int LT, int LU, int LV, double density
     elp_lin_host* h = (elp_lin_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.polLength = N; h->sp.density = density; h->sp.seed = seed; h->sp.nDevices = 1;
-    h->sp.boxUnit = box_unit;
-    h->sp.sawSiteFraction = site_fraction > 0 ? site_fraction : ELP_SAW_SITE_FRACTION;
-    SetBoxDimension(&h->sp, LT, LU, LV);
+    SetBoxDimensionUnit(&h->sp, LT, LU, LV, box_unit ? box_unit : ELP_LIN_TILE);
+    if (!h->sp.L) { elp_lin_host_free(h); return NULL; }
     SimStateInit(&h->ss, &h->sp);
-    if (BuildSAWPolymers(&h->sp, &h->ss, N, density, h->sp.sawSiteFraction)) { elp_lin_host_free(h); return NULL; }
+    const double frac = site_fraction > 0 ? site_fraction : ELP_SAW_SITE_FRACTION;
+    if (BuildSAWPolymers(&h->sp, &h->ss, N, density, frac)) { elp_lin_host_free(h); return NULL; }
     RedistribSL(&h->ss, &h->sp);
     if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { elp_lin_host_free(h); return NULL; }
     return h;
 }
 
-elp_lin_host* elp_lin_host_from_file(const char* path, uint32_t seed) {
+/* Start from a tuv file whose box edges are multiples of box_unit (a multiple
+   of 4); NULL on a bad or unreadable file. */
+elp_lin_host* elp_lin_host_from_file_unit(const char* path, uint32_t seed, int box_unit) {
+    if (box_unit <= 0 || box_unit % 4) return NULL;
     elp_lin_host* h = (elp_lin_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.seed = seed; h->sp.nDevices = 1;
-    if (ReadLatticeFile(&h->ss, &h->sp, path) || UpdatePolymerWithLabels(&h->ss) || GetAllRingPolymers(&h->sp, &h->ss) ||
-        UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    if (ReadLatticeFileUnit(&h->ss, &h->sp, path, box_unit) || UpdatePolymerWithLabels(&h->ss) ||
+        GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss) || !h->ss.nPol) {
+        elp_lin_host_free(h);
+        return NULL;
+    }
     return h;
+}
+
+/* elp_lin_host_from_file_unit with the largest unit of 32, 16, 8, 4 dividing
+   the file's box edges (32 for every box the reference writes) */
+elp_lin_host* elp_lin_host_from_file(const char* path, uint32_t seed) {
+    int L[3] = {0, 0, 0};
+    FILE* f = fopen(path, "r");
+    if (!f) return NULL;
+    int ok = fscanf(f, "%*s %i", L) == 1 && fscanf(f, "%*s %i", L + 1) == 1 && fscanf(f, "%*s %i", L + 2) == 1;
+    fclose(f);
+    if (!ok) return NULL;
+    for (int unit = ELP_LIN_TILE; unit >= 4; unit /= 2)
+        if (L[0] > 0 && L[1] > 0 && L[2] > 0 && !(L[0] % unit) && !(L[1] % unit) && !(L[2] % unit))
+            return elp_lin_host_from_file_unit(path, seed, unit);
+    return NULL;
 }
 
 void elp_lin_host_free(elp_lin_host* h) {

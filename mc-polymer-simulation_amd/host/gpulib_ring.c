@@ -17,6 +17,7 @@
 #include <string.h>
 #include "../../include/elp_gpu.h"
 #include "gpupol_ring.h"
+#include "elp_host_common.h"
 
 #define ELPCHK(call)                                                                  \
     do {                                                                              \

@@ -41,7 +41,7 @@ int main(int argc, char** argv) {
     if (!sp.LU) sp.LU = sp.LT;
     if (!sp.LV) sp.LV = sp.LT;
 
-    LoadPolymers(&sp, ss, devices, &gpuContext);
+    LoadPolymers(&sp, ss);
     GPULibLoadBuffers(&sp, ss, devices, &gpuContext);
 
     const char* clean = "\b \b\b \b\b \b\b \r";

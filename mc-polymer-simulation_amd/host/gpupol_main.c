@@ -43,13 +43,11 @@ int main(int argc, char** argv) {
         fprintf(stderr, "usage: %s NMONO TIME SEED DIR [DENSITY FAST_EQ INTERVAL LT LU LV EQUILIBRATED DBL_STEP]\n", argv[0]);
         return 2;
     }
-    const char* init = getenv("ELP_LIN_INIT");
-    if (init && !strncmp(init, "saw", 3)) sp.sawSiteFraction = init[3] == ':' ? atof(init + 4) : ELP_SAW_SITE_FRACTION;
     if (!sp.LT) sp.LT = 32;
     if (!sp.LU) sp.LU = sp.LT;
     if (!sp.LV) sp.LV = sp.LT;
     printf("Box: %ix%ix%i\n", sp.LT, sp.LU, sp.LV);
-    LoadPolymers(&sp, ss, devices, &gpuContext);
+    LoadPolymers(&sp, ss);
     GPULibLoadBuffers(&sp, ss, devices, &gpuContext);
     const char* clean = "\b \b\b \b\b \b\b \r";
     double totStep = (double)(sp.tMax - sp.curT);

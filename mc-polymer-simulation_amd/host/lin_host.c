@@ -10,6 +10,7 @@
 #include <string.h>
 #include <sys/stat.h>
 #include "gpupol_lin.h"
+#include "elp_host_common.h"
 
 uint SiteIndex(uint t, uint u, uint v, const SimProperties* sp) { return t + sp->LT * (u + sp->LU * v); }
 
@@ -56,9 +57,10 @@ void SimStateFree(SimState* ss) {
     memset(ss, 0, sizeof *ss);
 }
 
-/* gpupol/gpupol_init.c:46-65 */
-void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
-    const int E = sp->boxUnit > 0 ? sp->boxUnit : ELP_LIN_TILE;   /* 0: the reference's 32 */
+/* gpupol/gpupol_init.c:46-65: box edges truncated to multiples of `unit`
+   (the reference: 32, the tile edge) */
+void SetBoxDimensionUnit(SimProperties* sp, int LT, int LU, int LV, int unit) {
+    const int E = unit > 0 ? unit : ELP_LIN_TILE;
     sp->nwt = LT / E; sp->nwu = LU / E; sp->nwv = LV / E;
     sp->nwg = (size_t)sp->nwt * sp->nwu * sp->nwv;
     sp->LT = E * sp->nwt; sp->LU = E * sp->nwu; sp->LV = E * sp->nwv;
@@ -67,6 +69,8 @@ void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
     sp->gpuLatSize = sp->L;
     sp->maxPolLength = sp->LT * sp->LU * 4;
 }
+
+void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) { SetBoxDimensionUnit(sp, LT, LU, LV, ELP_LIN_TILE); }
 
 /* gpupol/gpupol_init.c:67-84 */
 int SimPropDefaultInit(SimProperties* sp) {
@@ -112,11 +116,11 @@ static void FindBestBoxSize(SimProperties* sp, int nPol, int polSize, int* dRes)
 }
 
 /* gpupol/gpupol_init.c:245-291: two-row hairpin of 2*dt sites ending in 0xf */
-static void ConstructBarPolymer(SimState* ss, SimProperties* sp, int polId, int length) {
+static int ConstructBarPolymer(SimState* ss, SimProperties* sp, int polId, int length) {
     int nt = (int)sp->LT / ((length + 1) / 2 - 1) + 1;
     int nu = (int)sp->LU / 2, nv = (int)sp->LV;
     int dt = (int)sp->LT / nt, du = 2, dv = 1;
-    if (polId >= nt * nu * nv) { printf("Error constructing polymers: Too little space!\n"); exit(120); }
+    if (polId >= nt * nu * nv) { printf("Error constructing polymers: Too little space!\n"); return 120; }
     int sl = length - dt * du * dv;
     int t = (polId % nt) * dt, u = ((polId / nt) % nu) * du, v = (polId / (nt * nu)) * dv;
     uint* b = (uint*)calloc((size_t)length + 2 * dt + 2, sizeof(uint));
@@ -128,14 +132,15 @@ static void ConstructBarPolymer(SimState* ss, SimProperties* sp, int polId, int 
     b[m++] = 0xf;
     for (int i = 0; i < sl; i++) { out[2 * i] = 0x0; out[2 * i + 1] = b[i]; }
     for (int i = 2 * sl; i < length; i++) out[i] = b[i - sl];
-    AddPolymerToSim(ss, sp, (uint)t, (uint)u, (uint)v, length, out);
+    int rc = AddPolymerToSim(ss, sp, (uint)t, (uint)u, (uint)v, length, out);
     free(b); free(out);
+    return rc;
 }
 
 /* gpupol/gpupol_init.c:293-360: the compact cube path of the ring builder,
    opened at the end (last bond 0xf). */
-static void ConstructCubePolymer(SimState* ss, SimProperties* sp, uint t0, uint u0, uint v0, uint dt, uint du,
-                                 uint dv, uint sl) {
+static int ConstructCubePolymer(SimState* ss, SimProperties* sp, uint t0, uint u0, uint v0, uint dt, uint du,
+                                uint dv, uint sl) {
     uint halfT = dt / 2, tUnit = 0x1;
     size_t cap = 2 * (size_t)dt * du * dv + 4;
     uint* b = (uint*)malloc(cap * sizeof(uint));
@@ -162,12 +167,13 @@ static void ConstructCubePolymer(SimState* ss, SimProperties* sp, uint t0, uint 
     for (i = 0; i < sl; i++) { out[m++] = 0x0; out[m++] = b[i]; }
     for (; i < n; i++) out[m++] = b[i];
     if (sl + dt * du * dv != m) { printf("Internal error: Failed to create polymer (%u vs %u)\n", dt * du * dv + sl, m); exit(1); }
-    AddPolymerToSim(ss, sp, t0, u0, v0, (int)m, out);
+    int rc = AddPolymerToSim(ss, sp, t0, u0, v0, (int)m, out);
     free(b); free(out);
+    return rc;
 }
 
 /* gpupol/gpupol_init.c:180-243 */
-void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, double density) {
+int GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, double density) {
     double nDouble = density * sp->L / (double)polLength;
     double dPol = fabs(nDouble - (int)(nDouble + 0.5));
     int nPol = (int)(nDouble + 0.5);
@@ -178,10 +184,10 @@ void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, 
     FindBestBoxSize(sp, nPol, polLength, dRes);
     uint dt = (uint)dRes[0], du = (uint)dRes[1], dv = (uint)dRes[2];
     printf("dt,du,dv = (%i,%i,%i)\n", dt, du, dv);
-    if (!dt || !du || !dv) { printf("Error: not enough space to put polymer into\n"); exit(192); }
+    if (!dt || !du || !dv) { printf("Error: not enough space to put polymer into\n"); return 192; }
     uint nt = sp->LT / dt, nu = sp->LU / du, nv = sp->LV / dv;
     printf("N=(%u,%u,%u) = %u >= %i\n", nt, nu, nv, nt * nu * nv, nPol);
-    if (nt * nu * nv < (uint)nPol) { printf("Error: not enough space to put polymer into\n"); exit(192); }
+    if (nt * nu * nv < (uint)nPol) { printf("Error: not enough space to put polymer into\n"); return 192; }
     printf("Stored length density = %lf\n", (polLength - (dt * du * dv)) / (double)polLength);
     uint sl = (uint)polLength - dt * du * dv;
     if (sl >= (uint)polLength) { printf("Internal error: Failed to find polymer partition\n"); exit(1); }
@@ -190,21 +196,28 @@ void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, 
     sp->maxPolLength = polLength + 1;
     if (polLength < 100) {
         for (int d = 0; d < sp->nDevices; d++)
-            for (int i = 0; i < nPol; i++) ConstructBarPolymer(ss + d, sp, i, polLength);
+            for (int i = 0; i < nPol; i++) {
+                int rc = ConstructBarPolymer(ss + d, sp, i, polLength);
+                if (rc) return rc;
+            }
     } else {
         int nAdded = 0;
         for (uint t = 0; t < nt * dt && nAdded < nPol; t += dt)
             for (uint u = 0; u < nu * du && nAdded < nPol; u += du)
                 for (uint v = 0; v < nv * dv && nAdded < nPol; v += dv) {
-                    for (int d = 0; d < sp->nDevices; d++) ConstructCubePolymer(ss + d, sp, t, u, v, dt, du, dv, sl);
+                    for (int d = 0; d < sp->nDevices; d++) {
+                        int rc = ConstructCubePolymer(ss + d, sp, t, u, v, dt, du, dv, sl);
+                        if (rc) return rc;
+                    }
                     nAdded++;
                 }
     }
     sp->maxPolLength = polLength + 1;
-    if (GetAllRingPolymers(sp, ss)) exit(193);
+    if (GetAllRingPolymers(sp, ss)) return 193;
     for (int d = 0; d < sp->nDevices; d++)
-        if (UpdatePolymerWithLabels(ss + d)) exit(123);
+        if (UpdatePolymerWithLabels(ss + d)) return 123;
     printf("------------------------------------------\n");
+    return 0;
 }
 
 static int UniqueLabel(int lab, int nLabelBit, int head, int headBit) {
@@ -217,7 +230,7 @@ static int UniqueLabel(int lab, int nLabelBit, int head, int headBit) {
 
 /* gpupol/gpupol_init.c:422-482: labels = head 1,0,0,0,0, 18 id bits (MSB
    first), then alternating 1,0,... */
-void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, int length, uint* bonds) {
+int AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, int length, uint* bonds) {
     Polymer* pol;
     if (!ss->nPol) {
         ss->pol = (Polymer*)malloc(sizeof(Polymer));
@@ -239,7 +252,12 @@ void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, in
     pol->labelStart = 0;
     int lab = (ss->nPol == 1) ? 0 : ss->id2Label[ss->nPol - 2] + 1;
     while (!UniqueLabel(lab, ss->labelBit, ss->labelHead, ss->headBit)) lab++;
-    if (lab > (1 << ss->labelBit)) printf("Ran out of labels, please make a smaller box/longer polymers!\n");
+    if (lab >= (1 << ss->labelBit)) {
+        printf("Ran out of labels, please make a smaller box/longer polymers!\n");
+        ss->nPol--;
+        free(pol->label); free(pol->bonds);
+        return 2;
+    }
     pol->label[0] = 1;
     for (int i = 1; i < ss->headBit; i++) pol->label[i] = 0;
     for (int i = ss->headBit; i < ss->headBit + ss->labelBit; i++)
@@ -248,16 +266,19 @@ void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, in
     ss->label2Id[lab] = ss->nPol - 1;
     int alt = 1;
     for (int i = ss->labelBit + ss->headBit; i < length; i++, alt ^= 1) pol->label[i] = alt;
-    SetBondVecs(ss, sp, t, u, v, bonds, (uint)length, pol->label);
+    ss->polNumTranslate[ss->nPol - 1] = ss->nPol - 1;
+    pol->length = 0;
+    pol->polCoor = NULL;
+    if (SetBondVecs(ss, sp, t, u, v, bonds, (uint)length, pol->label)) return 1;
     memcpy(pol->bonds, bonds, sizeof(uint) * length);
     pol->length = length;
-    ss->polNumTranslate[ss->nPol - 1] = ss->nPol - 1;
     pol->startTUV.t = t; pol->startTUV.u = u; pol->startTUV.v = v;
     pol->startLeftSL = 0;
+    return 0;
 }
 
 /* gpupol/oclpol.c:126-162 (the chain-end monomer carries no label bit) */
-void SetBondVecs(SimState* ss, SimProperties* sp, uint t, uint u, uint v, uint* bonds, uint polSize, uint* labels) {
+int SetBondVecs(SimState* ss, SimProperties* sp, uint t, uint u, uint v, uint* bonds, uint polSize, uint* labels) {
     uint16_t* lat = ss->lattice;
     uint s = SiteIndex(t, u, v, sp);
     for (uint i = 0; i < polSize; i++) {
@@ -266,14 +287,15 @@ void SetBondVecs(SimState* ss, SimProperties* sp, uint t, uint u, uint v, uint* 
         } else if (bonds[i] == 0xf) {
             break;
         } else {
-            if (LW_NEXT(lat[s])) { printf("Error setting bond vec: site already occupied\n(%i %i %i)\n", t, u, v); exit(1); }
+            if (LW_NEXT(lat[s])) { printf("Error setting bond vec: site already occupied\n(%i %i %i)\n", t, u, v); return 1; }
             lat[s] |= (uint16_t)(bonds[i] | ((labels[i] & 1) << 9));
             AddUnitTUV(bonds[i], &t, &u, &v, sp);
             s = SiteIndex(t, u, v, sp);
-            if (LW_PREV(lat[s])) { printf("Error setting bond vec: site already occupied[2]\n(%i %i %i)\n", t, u, v); exit(1); }
+            if (LW_PREV(lat[s])) { printf("Error setting bond vec: site already occupied[2]\n(%i %i %i)\n", t, u, v); return 1; }
             lat[s] |= (uint16_t)(bonds[i] << 4);
         }
     }
+    return 0;
 }
 
 /* gpupol/gpupol_init.c:484-532 */
@@ -410,21 +432,23 @@ int UpdatePolymerWithLabels(SimState* ss) {
     return fail ? -1 : 0;
 }
 
-/* gpupol/gpupol_io.c:91-149 */
-int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
+/* gpupol/gpupol_io.c:91-149; box edges must be multiples of `unit` (the
+   reference: 32), else 202 is returned */
+int ReadLatticeFileUnit(SimState* ss, SimProperties* sp, const char* file, int unit) {
     FILE* f = fopen(file, "r");
     if (!f) { printf("Error reading file %s\n", file); return 199; }
     int LT, LU, LV, np = 0, maxLen = 0;
     if (fscanf(f, "%*s %i", &LT) != 1 || fscanf(f, "%*s %i", &LU) != 1 || fscanf(f, "%*s %i", &LV) != 1) {
         fclose(f); return 199;
     }
-    const int E = ELP_LIN_TILE;
-    if (LT % E || LU % E || LV % E) {
+    const int E = unit > 0 ? unit : ELP_LIN_TILE;
+    if (LT <= 0 || LU <= 0 || LV <= 0 || LT % E || LU % E || LV % E) {
         printf("Error: Box dimensions not correct. Work group dimensions: (%i, %i, %i) vs file (%i, %i, %i)\n",
                E, E, E, LT, LU, LV);
-        exit(202);
+        fclose(f);
+        return 202;
     }
-    SetBoxDimension(sp, LT, LU, LV);
+    SetBoxDimensionUnit(sp, LT, LU, LV, E);
     SimStateInit(ss, sp);
     if (fscanf(f, "%*s %i", &np) != 1 || fscanf(f, "%*s %i", &maxLen) != 1 || np < 0 || maxLen <= 0) { fclose(f); return 199; }
     sp->maxPolLength = (uint)maxLen;
@@ -442,13 +466,24 @@ int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
         for (int i = 0; i < len; i++) {
             char c = text[i];
             bonds[i] = (c >= '0' && c <= '9') ? (uint)(c - '0') : (uint)(10 + (c | 0x20) - 'a');
+            if (bonds[i] > 0xf) {
+                printf("Error: bad bond character in %s\n", file);
+                fclose(f); free(bonds); free(text); return 199;
+            }
         }
-        AddPolymerToSim(ss, sp, t, u, v, len, bonds);
+        if (AddPolymerToSim(ss, sp, t, u, v, len, bonds)) {
+            printf("Error: polymer %i of %s overlaps another one\n", p, file);
+            fclose(f); free(bonds); free(text); return 199;
+        }
     }
     fclose(f);
     free(bonds); free(text);
     printf("Succesfully read file %s\n", file);
     return 0;
+}
+
+int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
+    return ReadLatticeFileUnit(ss, sp, file, ELP_LIN_TILE);
 }
 
 /* gpupol/gpupol_io.c:151-187 */
@@ -494,10 +529,21 @@ static void mkdir_p(const char* path) {
     mkdir(buf, 0777);
 }
 
-/* gpupol/gpupol_init.c:86-150 */
-void LoadPolymers(SimProperties* sp, SimState* ss, GPUDeviceState* devices, GPULibContext* ctx) {
+/* Fraction of the sites a fresh self-avoiding-walk start covers, from
+   ELP_LIN_INIT ("saw" or "saw:FRACTION"); 0 = the reference's start. */
+static double saw_fraction_from_env(void) {
+    const char* init = getenv("ELP_LIN_INIT");
+    if (!init || strncmp(init, "saw", 3)) return 0;
+    return init[3] == ':' ? atof(init + 4) : ELP_SAW_SITE_FRACTION;
+}
+
+/* gpupol/gpupol_init.c:86-150.  The device context of GPULibInit stays the
+   library's own: one device, index 0. */
+void LoadPolymers(SimProperties* sp, SimState* ss) {
     char file[8192];
-    GPULibInit(sp, devices, ctx);
+    GPULibContext ctx;
+    GPULibInit(sp, NULL, &ctx);
+    const double sawFraction = saw_fraction_from_env();
     mkdir_p(sp->dirOut);
     long highT = -1;
     int nHigh = -1;
@@ -517,16 +563,18 @@ void LoadPolymers(SimProperties* sp, SimState* ss, GPUDeviceState* devices, GPUL
         sp->tMax += highT;
         for (int i = 0; i < sp->nDevices; i++) {
             snprintf(file, sizeof file, "%s/%s", sp->dirOut, highFiles[i]);
-            if (ReadLatticeFile(ss + i, sp, file)) exit(199);
+            int rc = ReadLatticeFile(ss + i, sp, file);
+            if (rc) exit(rc);
             if (UpdatePolymerWithLabels(ss + i)) exit(123);
         }
     } else {
         SetBoxDimension(sp, (int)sp->LT, (int)sp->LU, (int)sp->LV);
         SimStateInit(ss, sp);
-        if (sp->sawSiteFraction > 0) {
-            if (BuildSAWPolymers(sp, ss, sp->polLength, sp->density, sp->sawSiteFraction)) exit(192);
+        if (sawFraction > 0) {
+            if (BuildSAWPolymers(sp, ss, sp->polLength, sp->density, sawFraction)) exit(192);
         } else {
-            GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+            int rc = GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+            if (rc) exit(rc);
         }
         RedistribSL(ss, sp);
         for (int i = 0; i < sp->nDevices; i++) {

@@ -29,6 +29,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "gpupol_lin.h"
+#include "elp_host_common.h"
 
 static const uint kSawDirs[12] = {0x1, 0x2, 0x4, 0x5, 0x6, 0x7, 0x8, 0x9, 0xa, 0xb, 0xd, 0xe};
 

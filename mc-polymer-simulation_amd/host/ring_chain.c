@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "gpupol_ring.h"
+#include "elp_host_common.h"
 
 #define SITE_BOND(x) ((x) & 0xf)
 #define SITE_SL(x) (((x) >> 6) & 0x1)
@@ -26,8 +27,8 @@ void AddUnitTUV(uint unit, uint* t, uint* u, uint* v, const SimProperties* sp) {
 }
 
 /* gpupol2/oclpol.c:157-181 */
-void SetBondVecs(uint8_t* lattice, uint t, uint u, uint v, uint* bonds, uint polSize, uint* labels,
-                 SimProperties* sp) {
+int SetBondVecs(uint8_t* lattice, uint t, uint u, uint v, uint* bonds, uint polSize, uint* labels,
+                SimProperties* sp) {
     uint site = SiteIndex(t, u, v, sp);
     for (uint i = 0; i < polSize; i++) {
         if (bonds[i] == 0x0) {
@@ -39,13 +40,14 @@ void SetBondVecs(uint8_t* lattice, uint t, uint u, uint v, uint* bonds, uint pol
             if (SITE_BOND(lattice[site])) {
                 printf("Error setting bond vector %i: site already occupied\n", i);
                 printf("(%i %i %i)\n", t, u, v);
-                exit(1);
+                return 1;
             }
             lattice[site] = (uint8_t)((lattice[site] & ~0x1f) | bonds[i] | ((labels[i] & 1) << 4));
             AddUnitTUV(bonds[i], &t, &u, &v, sp);
             site = SiteIndex(t, u, v, sp);
         }
     }
+    return 0;
 }
 
 /* gpupol2/oclpol.c:146-155 */

@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "gpupol_ring.h"
+#include "elp_host_common.h"
 
 /* gpupol2/gpupol_init.c:3-37 */
 int SimStateInit(SimState* ss, SimProperties* sp) {
@@ -53,10 +54,10 @@ void SimStateFree(SimState* ss) {
 /* gpupol2/gpupol_init.c:39-41 */
 void ClearLattice(SimState* ss, SimProperties* sp) { memset(ss->lattice, 0, sp->latSize); }
 
-/* gpupol2/gpupol_init.c:43-62 -- box edges truncated to multiples of 32
-   (of sp->boxUnit when set: boxes such as 48 for tiles of 16) */
-void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
-    const int E = sp->boxUnit > 0 ? sp->boxUnit : ELP_RING_TILE;
+/* gpupol2/gpupol_init.c:43-62 -- box edges truncated to multiples of `unit`
+   (the reference: 32, the tile edge; 16 allows boxes such as 48 for tiles of 16) */
+void SetBoxDimensionUnit(SimProperties* sp, int LT, int LU, int LV, int unit) {
+    const int E = unit > 0 ? unit : ELP_RING_TILE;
     if (LT % E || LU % E || LV % E) printf("Warning changing box dimensions to be a multiple of 24!\n");
     sp->nwt = LT / E; sp->nwu = LU / E; sp->nwv = LV / E;
     sp->nwg = (size_t)sp->nwt * sp->nwu * sp->nwv;
@@ -65,6 +66,8 @@ void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) {
     sp->latSize = sp->L;
     sp->maxPolLength = sp->LT * sp->LU * 4;
 }
+
+void SetBoxDimension(SimProperties* sp, int LT, int LU, int LV) { SetBoxDimensionUnit(sp, LT, LU, LV, ELP_RING_TILE); }
 
 /* gpupol2/gpupol_init.c:64-81 */
 int SimPropDefaultInit(SimProperties* sp) {
@@ -111,7 +114,7 @@ static void FindBestBoxSize(SimProperties* sp, int nPol, int polSize, int* dRes)
 }
 
 /* gpupol2/gpupol_init.c:183-248 */
-void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, double density) {
+int GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, double density) {
     double nDouble = density * sp->L / (double)polLength;
     double dPol = fabs(nDouble - (int)(nDouble + 0.5));
     int nPol = (int)(nDouble + 0.5);
@@ -122,10 +125,10 @@ void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, 
     FindBestBoxSize(sp, nPol, polLength, dRes);
     uint dt = (uint)dRes[0], du = (uint)dRes[1], dv = (uint)dRes[2];
     printf("dt,du,dv = (%i,%i,%i)\n", dt, du, dv);
-    if (!dt || !du || !dv) { printf("Error: not enough space to put polymer into\n"); exit(192); }
+    if (!dt || !du || !dv) { printf("Error: not enough space to put polymer into\n"); return 192; }
     uint nt = sp->LT / dt, nu = sp->LU / du, nv = sp->LV / dv;
     printf("N=(%u,%u,%u) = %u >= %i\n", nt, nu, nv, nt * nu * nv, nPol);
-    if (nt * nu * nv < (uint)nPol) { printf("Error: not enough space to put polymer into\n"); exit(192); }
+    if (nt * nu * nv < (uint)nPol) { printf("Error: not enough space to put polymer into\n"); return 192; }
     printf("Stored length density = %lf\n", (polLength - (dt * du * dv)) / (double)polLength);
     uint sl = (uint)polLength - dt * du * dv;
     if (sl >= (uint)polLength) { printf("Internal error: Failed to find polymer partition\n"); exit(1); }
@@ -137,20 +140,25 @@ void GetBestPolymerPartitioning(SimProperties* sp, SimState* ss, int polLength, 
     for (uint t = 0; t < nt * dt && nAdded < nPol; t += sp->LT / nt)
         for (uint u = 0; u < nu * du && nAdded < nPol; u += sp->LU / nu)
             for (uint v = 0; v < nv * dv && nAdded < nPol; v += sp->LV / nv) {
-                for (int d = 0; d < sp->nDevices; d++) ConstructCubeRingPolymer(ss + d, sp, t, u, v, dt, du, dv, sl);
+                for (int d = 0; d < sp->nDevices; d++) {
+                    int rc = ConstructCubeRingPolymer(ss + d, sp, t, u, v, dt, du, dv, sl);
+                    if (rc) return rc;
+                }
                 nAdded++;
             }
     sp->maxPolLength = polLength + 1;
-    GetAllRingPolymers(sp, ss);
-    for (int d = 0; d < sp->nDevices; d++) UpdatePolymerWithLabels(ss + d);
+    if (GetAllRingPolymers(sp, ss)) return 193;
+    for (int d = 0; d < sp->nDevices; d++)
+        if (UpdatePolymerWithLabels(ss + d)) return 123;
     printf("------------------------------------------\n");
+    return 0;
 }
 
 /* gpupol2/gpupol_init.c:298-365: a compact ring filling a dt x du x dv
    block (boustrophedon in t/u, stacked and mirrored in v), then sl
    stored-length monomers inserted in front of the first sl bonds. */
-void ConstructCubeRingPolymer(SimState* ss, SimProperties* sp, uint tStart, uint uStart, uint vStart,
-                              uint dt, uint du, uint dv, uint sl) {
+int ConstructCubeRingPolymer(SimState* ss, SimProperties* sp, uint tStart, uint uStart, uint vStart,
+                             uint dt, uint du, uint dv, uint sl) {
     uint halfT = dt / 2;
     uint tUnit = 0x1;
     const uint uUnit = 0x2, vUnit = 0x4;
@@ -187,8 +195,9 @@ void ConstructCubeRingPolymer(SimState* ss, SimProperties* sp, uint tStart, uint
                dt * du * dv + sl, m, n, sl);
         exit(1);
     }
-    AddPolymerToSim(ss, sp, tStart, uStart, vStart, (int)m, out);
+    int rc = AddPolymerToSim(ss, sp, tStart, uStart, vStart, (int)m, out);
     free(b); free(out);
+    return rc;
 }
 
 /* gpupol2/gpupol_init.c:415-425: does the 18-bit label create a false head? */
@@ -202,7 +211,7 @@ int UniqueLabel(int startLabel, int nLabelBit, int head, int headBit) {
 
 /* gpupol2/gpupol_init.c:427-499: labels = 18 id bits (MSB first),
    alternating 1,0,... and the head 1,0,0,0,0 at the end. */
-void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, int length, uint* bonds) {
+int AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, int length, uint* bonds) {
     Polymer* pol;
     if (!ss->nPol) {
         ss->pol = (Polymer*)malloc(sizeof(Polymer));
@@ -224,7 +233,12 @@ void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, in
     pol->labelStart = 0;
     int lab = (ss->nPol == 1) ? 0 : ss->id2Label[ss->nPol - 2] + 1;
     while (!UniqueLabel(lab, ss->labelBit, ss->labelHead, ss->headBit)) lab++;
-    if (lab > (1 << ss->labelBit)) printf("Ran out of labels, please make a smaller box/longer polymers!\n");
+    if (lab >= (1 << ss->labelBit)) {
+        printf("Ran out of labels, please make a smaller box/longer polymers!\n");
+        ss->nPol--;
+        free(pol->label); free(pol->bonds);
+        return 2;
+    }
     for (int i = 0; i < ss->labelBit; i++) pol->label[i] = (lab >> (ss->labelBit - 1 - i)) & 0x1;
     int alt = 1;
     for (int i = ss->labelBit; i < length - ss->headBit; i++, alt ^= 1) pol->label[i] = alt;
@@ -232,12 +246,14 @@ void AddPolymerToSim(SimState* ss, SimProperties* sp, uint t, uint u, uint v, in
     for (int i = length - ss->headBit + 1; i < length; i++) pol->label[i] = 0;
     ss->id2Label[ss->nPol - 1] = lab;
     ss->label2Id[lab] = ss->nPol - 1;
-    SetBondVecs(ss->lattice, t, u, v, bonds, (uint)length, pol->label, sp);
+    ss->polNumTranslate[ss->nPol - 1] = ss->nPol - 1;
+    pol->length = 0;
+    if (SetBondVecs(ss->lattice, t, u, v, bonds, (uint)length, pol->label, sp)) return 1;
     memcpy(pol->bonds, bonds, sizeof(uint) * length);
     pol->length = length;
-    ss->polNumTranslate[ss->nPol - 1] = ss->nPol - 1;
     pol->startTUV.t = t; pol->startTUV.u = u; pol->startTUV.v = v;
     pol->startLeftSL = 0;
+    return 0;
 }
 
 /* gpupol2/gpupol_init.c:501-550: uniform reshuffle of the stored-length

@@ -13,9 +13,11 @@
 #include <string.h>
 #include <sys/stat.h>
 #include "gpupol_ring.h"
+#include "elp_host_common.h"
 
-/* gpupol2/gpupol_io.c:105-165 */
-int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
+/* gpupol2/gpupol_io.c:105-165; box edges must be multiples of `unit` (the
+   reference: 32, :131-134), else 202 is returned */
+int ReadLatticeFileUnit(SimState* ss, SimProperties* sp, const char* file, int unit) {
     printf("Reading lattice file\n");
     FILE* f = fopen(file, "r");
     if (!f) { printf("Error reading file %s\n", file); return 199; }
@@ -23,13 +25,14 @@ int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
     if (fscanf(f, "%*s %i", &LT) != 1 || fscanf(f, "%*s %i", &LU) != 1 || fscanf(f, "%*s %i", &LV) != 1) {
         printf("Error: malformed header in %s\n", file); fclose(f); return 199;
     }
-    const int E = ELP_RING_TILE;
-    if (LT % E || LU % E || LV % E) {
+    const int E = unit > 0 ? unit : ELP_RING_TILE;
+    if (LT <= 0 || LU <= 0 || LV <= 0 || LT % E || LU % E || LV % E) {
         printf("Error: Box dimensions not correct. Work group dimensions: (%i, %i, %i) vs file (%i, %i, %i)\n",
                E, E, E, LT, LU, LV);
-        exit(202);
+        fclose(f);
+        return 202;
     }
-    SetBoxDimension(sp, LT, LU, LV);
+    SetBoxDimensionUnit(sp, LT, LU, LV, E);
     SimStateInit(ss, sp);
     if (fscanf(f, "%*s %i", &np) != 1 || fscanf(f, "%*s %i", &maxLen) != 1 || np < 0 || maxLen <= 0) {
         printf("Error: malformed header in %s\n", file); fclose(f); return 199;
@@ -51,13 +54,20 @@ int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
             bonds[i] = (c >= '0' && c <= '9') ? (uint)(c - '0') : (uint)(10 + (c | 0x20) - 'a');
             if (bonds[i] > 0xf) { printf("Error: bad bond character in %s\n", file); fclose(f); free(bonds); free(text); return 199; }
         }
-        AddPolymerToSim(ss, sp, t, u, v, len, bonds);
+        if (AddPolymerToSim(ss, sp, t, u, v, len, bonds)) {
+            printf("Error: polymer %i of %s overlaps another one\n", p, file);
+            fclose(f); free(bonds); free(text); return 199;
+        }
     }
     fclose(f);
     free(bonds); free(text);
     printf("Succesfully read file %s\n", file);
     printf("Finished Reading lattice file\n");
     return 0;
+}
+
+int ReadLatticeFile(SimState* ss, SimProperties* sp, const char* file) {
+    return ReadLatticeFileUnit(ss, sp, file, ELP_RING_TILE);
 }
 
 /* gpupol2/gpupol_io.c:191-203 */
@@ -116,10 +126,12 @@ static void mkdir_p(const char* path) {
 
 /* gpupol2/gpupol_init.c:83-152: resume from the highest t=<T>_dev=<i>.res
    files if there is one per device (seed += T/interval), else build the
-   compact rings, reshuffle stored length and write t=0. */
-void LoadPolymers(SimProperties* sp, SimState* ss, GPUDeviceState* devices, GPULibContext* ctx) {
+   compact rings, reshuffle stored length and write t=0.  The device context
+   of GPULibInit stays the library's own: one device, index 0. */
+void LoadPolymers(SimProperties* sp, SimState* ss) {
     char file[8192];
-    GPULibInit(sp, devices, ctx);
+    GPULibContext ctx;
+    GPULibInit(sp, NULL, &ctx);
     mkdir_p(sp->dirOut);
     long highT = -1;
     int nHigh = -1;
@@ -147,13 +159,15 @@ void LoadPolymers(SimProperties* sp, SimState* ss, GPUDeviceState* devices, GPUL
         for (int i = 0; i < sp->nDevices; i++) {
             snprintf(file, sizeof file, "%s/%s", sp->dirOut, highFiles[i]);
             printf("file = %s\n", file);
-            if (ReadLatticeFile(ss + i, sp, file)) exit(199);
+            int rc = ReadLatticeFile(ss + i, sp, file);
+            if (rc) exit(rc);
             if (UpdatePolymerWithLabels(ss + i)) exit(123);
         }
     } else {
         SetBoxDimension(sp, (int)sp->LT, (int)sp->LU, (int)sp->LV);
         SimStateInit(ss, sp);
-        GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+        int rc = GetBestPolymerPartitioning(sp, ss, sp->polLength, sp->density);
+        if (rc) exit(rc);
         RedistribSL(ss, sp);
         for (int i = 0; i < sp->nDevices; i++) {
             snprintf(file, sizeof file, "%s/t=0_dev=%i.res", sp->dirOut, i);

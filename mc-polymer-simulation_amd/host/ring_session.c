@@ -9,12 +9,13 @@
 #include <stdlib.h>
 #include <string.h>
 #include "gpupol_ring.h"
+#include "elp_host_common.h"
 
-typedef struct elp_ring_host {
+struct elp_ring_host {
     SimProperties sp;
     SimState ss;
     long cumT, cumU, cumV;
-} elp_ring_host;
+};
 
 /* Fresh start (no files): box edges truncated to multiples of box_unit (0: 32,
    as the reference; a smaller unit, e.g. 16, allows boxes such as 48 for
@@ -28,12 +29,12 @@ elp_ring_host* elp_ring_host_new_unit(int N, int LT, int LU, int LV, double dens
     h->sp.density = density;
     h->sp.seed = seed;
     h->sp.nDevices = 1;
-    h->sp.boxUnit = box_unit;
-    SetBoxDimension(&h->sp, LT, LU, LV);
+    SetBoxDimensionUnit(&h->sp, LT, LU, LV, box_unit ? box_unit : ELP_RING_TILE);
+    if (!h->sp.L) { elp_ring_host_free(h); return NULL; }
     SimStateInit(&h->ss, &h->sp);
-    GetBestPolymerPartitioning(&h->sp, &h->ss, N, density);
+    if (GetBestPolymerPartitioning(&h->sp, &h->ss, N, density)) { elp_ring_host_free(h); return NULL; }
     RedistribSL(&h->ss, &h->sp);
-    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { elp_ring_host_free(h); return NULL; }
     return h;
 }
 
@@ -42,16 +43,38 @@ elp_ring_host* elp_ring_host_new(int N, int LT, int LU, int LV, double density, 
     return elp_ring_host_new_unit(N, LT, LU, LV, density, seed, 0);
 }
 
-/* Start from a tuv file (resume path of LoadPolymers, without the seed bump). */
-elp_ring_host* elp_ring_host_from_file(const char* path, uint32_t seed) {
+/* Start from a tuv file (resume path of LoadPolymers, without the seed bump);
+   box edges must be multiples of box_unit (a multiple of 4).  Returns NULL on
+   a bad or unreadable file. */
+elp_ring_host* elp_ring_host_from_file_unit(const char* path, uint32_t seed, int box_unit) {
+    if (box_unit <= 0 || box_unit % 4) return NULL;
     elp_ring_host* h = (elp_ring_host*)calloc(1, sizeof *h);
     SimPropDefaultInit(&h->sp);
     h->sp.seed = seed;
     h->sp.nDevices = 1;
-    if (ReadLatticeFile(&h->ss, &h->sp, path) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
-    if (GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss)) { free(h); return NULL; }
+    if (ReadLatticeFileUnit(&h->ss, &h->sp, path, box_unit) || UpdatePolymerWithLabels(&h->ss) ||
+        GetAllRingPolymers(&h->sp, &h->ss) || UpdatePolymerWithLabels(&h->ss) || !h->ss.nPol) {
+        elp_ring_host_free(h);
+        return NULL;
+    }
     h->sp.polLength = h->ss.pol[0].length;
     return h;
+}
+
+/* elp_ring_host_from_file_unit with the largest unit of 32, 16, 8, 4 that
+   divides the file's box edges (32 for every box the reference writes, which
+   keeps its seed-table size and so the host stream position) */
+elp_ring_host* elp_ring_host_from_file(const char* path, uint32_t seed) {
+    int L[3] = {0, 0, 0};
+    FILE* f = fopen(path, "r");
+    if (!f) return NULL;
+    int ok = fscanf(f, "%*s %i", L) == 1 && fscanf(f, "%*s %i", L + 1) == 1 && fscanf(f, "%*s %i", L + 2) == 1;
+    fclose(f);
+    if (!ok) return NULL;
+    for (int unit = ELP_RING_TILE; unit >= 4; unit /= 2)
+        if (L[0] > 0 && L[1] > 0 && L[2] > 0 && !(L[0] % unit) && !(L[1] % unit) && !(L[2] % unit))
+            return elp_ring_host_from_file_unit(path, seed, unit);
+    return NULL;
 }
 
 void elp_ring_host_free(elp_ring_host* h) {
@@ -137,7 +160,7 @@ int elp_ring_host_label2id(const elp_ring_host* h, int32_t* out, uint64_t n) {
 /* The per-lane seed table drawn by SimStateInit (gpupol2/gpupol_init.c:22-33):
    (nwg * 512 * 8) uint32, lane l of work-group w at 8*(l + 512*w): 4 words of
    its own state, then 4 words shared by the work-group.  The counter-based
-   engine does not use it; it is exposed for reference-kernel comparisons. */
+   engine does not use it; drawing it keeps the host stream position. */
 int elp_ring_host_seeds(const elp_ring_host* h, uint32_t* out, uint64_t n) {
     uint64_t need = (uint64_t)h->sp.nwg * h->sp.ws * 2 * h->sp.R;
     if (n != need) return -1;
