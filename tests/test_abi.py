@@ -33,6 +33,28 @@ def test_host_libraries_export_gpulib_boundary():
     assert all("elp_lin_host_" + s in lin for s in HOST)
 
 
+def _c_functions(header):
+    """Names of the functions a C header declares (prototypes ending in ');')."""
+    import re
+    text = re.sub(r"/\*.*?\*/", "", open(header).read(), flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"#[^\n]*", "", text)
+    protos = re.findall(r"[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(([^;{}()]|\([^()]*\))*\)\s*;", text)
+    return sorted({name for name, _ in protos} - {"sizeof"})
+
+
+def test_host_libraries_export_every_declared_function():
+    """Every function include/gpupol_ring.h and include/gpupol_lin.h declare is
+    exported by libgpupol3_ring.so / libgpupol_lin.so."""
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    for header, lib in (("gpupol_ring.h", "libgpupol3_ring.so"), ("gpupol_lin.h", "libgpupol_lin.so")):
+        declared = _c_functions(os.path.join(inc, header))
+        assert "GPULibRun" in declared and "LoadPolymers" in declared and len(declared) > 30, declared
+        exported = set(elp.exported_symbols(os.path.join(LIB, lib)))
+        missing = [f for f in declared if f not in exported]
+        assert not missing, (header, missing)
+
+
 def test_error_paths_without_gpu_calls():
     """Argument validation happens before any device call."""
     lib = elp.load_elp()
