@@ -239,6 +239,20 @@ int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int 
 int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
 
+/* elp_ring_frame_summary without allocation.  No reference counterpart: for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving or analysis loops).  Same arguments and results; every
+   device buffer of the computation (label table, ring starts, counters, bond
+   strings, per-ring results, contact histogram) is carved from one static
+   32 MiB device array per GPU, so n_pol rings of up to max_len monomers must
+   fit: about 1 MiB + n_pol * (max_len + 72) + 8 * max_len bytes (ELP_E_ARG
+   beyond that; e.g. L=128 N=1000, 2517 rings, takes 3.8 MB).  The array is
+   shared by every call on that GPU and keeps the last call's contents (what
+   must start at zero is cleared by a kernel); calls that overlap on one GPU
+   (other threads, other engines' streams) race. */
+int elp_ring_frame_summary_noalloc(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                                   int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
+
 /*
  * Learned proposal for ring equilibration (no reference counterpart).
  * A small network (144 int8 features of a site's 5^3 neighbourhood and bond

@@ -26,5 +26,9 @@ constexpr int kSummaryMaxLen = 8192;
 constexpr size_t summary_lds_bytes(int n_max) { return 32 + 4 * 3 * 256 + 16 * (size_t)n_max; }
 int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int64_t* sum3,
                           int32_t* mid3, unsigned long long* contacts, hipStream_t st);
+// p[0..n) = v (clears scratch memory without a memset node)
+int chains_launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// stat[0] = longest chain (atomicMax), stat[1] += chains with no length (lens <= 0)
+int chains_launch_check_lens(const int32_t* lens, int n_pol, uint32_t* stat, hipStream_t st);
 
 }  // namespace elp

@@ -327,6 +327,31 @@ int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, 
     return (int)hipGetLastError();
 }
 
+__global__ void fill32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+int chains_launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
+    if (!n) return 0;
+    size_t nb = (n + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(fill32_kernel, dim3((unsigned)nb), dim3(256), 0, st, p, v, n);
+    return (int)hipGetLastError();
+}
+
+__global__ void check_lens_kernel(const int32_t* __restrict__ lens, int n_pol, uint32_t* __restrict__ stat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pol) return;
+    const int n = lens[i];
+    if (n <= 0) atomicAdd(&stat[1], 1u);
+    else atomicMax(&stat[0], (uint32_t)n);
+}
+
+int chains_launch_check_lens(const int32_t* lens, int n_pol, uint32_t* stat, hipStream_t st) {
+    hipLaunchKernelGGL(check_lens_kernel, dim3((n_pol + 255) / 256), dim3(256), 0, st, lens, n_pol, stat);
+    return (int)hipGetLastError();
+}
+
 int chains_launch_heads(const uint8_t* lat, int LT, int LU, int LV, uint32_t* starts, uint32_t cap, uint32_t* count,
                         hipStream_t st) {
     const uint32_t n = (uint32_t)LT * LU * LV;

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 #include <math.h>
+#include <mutex>
 #include "../../include/elp_gpu.h"
 #include "elp_chains.h"
 #include "elp_kernels.h"
@@ -16,15 +17,14 @@
 
 namespace {
 
-thread_local std::string g_err = "";
+// last error message of this thread (a fixed buffer: reporting an error allocates nothing)
+thread_local char g_err[512] = "";
 
 int fail(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -143,7 +143,7 @@ static int create(elp_engine** out, int kind, int device, int LT, int LU, int LV
 
 extern "C" {
 
-const char* elp_last_error(void) { return g_err.c_str(); }
+const char* elp_last_error(void) { return g_err; }
 
 int elp_device_count(int* n) {
     if (!n) return fail(ELP_E_ARG, "n is NULL");
@@ -981,6 +981,64 @@ int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, in
     return ELP_OK;
 }
 
+// ---- frame summary from a static scratch array (no allocation) ----
+// Every device buffer of elp_ring_frame_summary_noalloc lives in `scratch`, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress returns for the current GPU, looked up once per device.
+constexpr size_t kFrameScratchBytes = (size_t)32 << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kFrameScratchBytes];
+static void* g_scratch_ptr[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+// device address of this GPU's scratch (the device becomes current)
+static int scratch_on(int device, void** out) {
+    if (device < 0 || device >= kMaxScratchDevices) return fail(ELP_E_ARG, "device %d out of range", device);
+    HIPCHK(hipSetDevice(device));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p) return hip_fail(err, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return ELP_OK;
+}
+
+struct FrameScratch {
+    int32_t* l2i;            // label -> polymer id (2^18)
+    uint32_t* starts;        // ring start cursors
+    uint32_t* stat;          // [0] heads found, [1] walk error bits, [2] longest ring, [3] ids not found
+    uint8_t* bonds;          // n_pol x max_len bond strings
+    int32_t *lens, *start3, *line_len, *nsl, *mid3;
+    double* rg2;
+    int64_t* sum3;
+    unsigned long long* contacts;
+};
+
+// Buffers of one frame summary carved from `base` in 256-byte aligned pieces;
+// returns the bytes they take (base = 0 only sizes them).
+static size_t frame_scratch_layout(uintptr_t base, int n_pol, int max_len, FrameScratch* S) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) { uintptr_t p = base + off; off += (bytes + 255) & ~(size_t)255; return p; };
+    const size_t n = (size_t)n_pol;
+    S->l2i = (int32_t*)take(sizeof(int32_t) << 18);
+    S->starts = (uint32_t*)take(sizeof(uint32_t) * n);
+    S->stat = (uint32_t*)take(sizeof(uint32_t) * 4);
+    S->bonds = (uint8_t*)take(n * (size_t)max_len);
+    S->lens = (int32_t*)take(sizeof(int32_t) * n);
+    S->start3 = (int32_t*)take(sizeof(int32_t) * 3 * n);
+    S->line_len = (int32_t*)take(sizeof(int32_t) * n);
+    S->nsl = (int32_t*)take(sizeof(int32_t) * n);
+    S->mid3 = (int32_t*)take(sizeof(int32_t) * 3 * n);
+    S->rg2 = (double*)take(sizeof(double) * n);
+    S->sum3 = (int64_t*)take(sizeof(int64_t) * 3 * n);
+    S->contacts = (unsigned long long*)take(sizeof(unsigned long long) * (size_t)max_len);
+    return off;
+}
+
 static int tuv_text(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len,
                     int header_max_pol_length, char* text, uint64_t cap, uint64_t* len) {
     if (!len) return fail(ELP_E_ARG, "len is NULL");
@@ -1009,6 +1067,61 @@ static int tuv_text(elp_engine* e, int kind, const int32_t* label2id, int n_pol,
     memcpy(text, head, (size_t)hn);
     HIPCHK(hipMemcpyAsync(text + hn, B.text.p, total, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return ELP_OK;
+}
+
+int elp_ring_frame_summary_noalloc(elp_engine* e, const int32_t* label2id, int n_pol, int max_len,
+                                   int32_t* start3, int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl,
+                                   uint64_t* contacts) {
+    if (!e || !label2id || n_pol <= 0 || max_len <= 0 || !start3 || !sum3 || !mid3)
+        return fail(ELP_E_ARG, "frame summary needs an engine, label2id, n_pol > 0, max_len > 0, start3, sum3, mid3");
+    if (e->kind != 0) return fail(ELP_E_ARG, "frame summary needs a ring engine");
+    if (!e->uploaded) return fail(ELP_E_STATE, "frame summary before upload");
+    FrameScratch S;
+    const size_t need = frame_scratch_layout(0, n_pol, max_len, &S);
+    if (need > kFrameScratchBytes)
+        return fail(ELP_E_ARG, "frame summary of %d rings of up to %d monomers needs %zu bytes of scratch; "
+                    "the static scratch holds %zu", n_pol, max_len, need, kFrameScratchBytes);
+    void* base = nullptr;
+    int rc = scratch_on(e->device, &base);
+    if (rc) return rc;
+    frame_scratch_layout((uintptr_t)base, n_pol, max_len, &S);
+    hipStream_t st = e->stream;
+    // the previous call's contents stay in the scratch: clear what is accumulated
+    if ((rc = elp::chains_launch_fill32(S.stat, 0u, 4, st)) ||
+        (rc = elp::chains_launch_fill32((uint32_t*)S.lens, 0u, (size_t)n_pol, st)) ||
+        (rc = elp::chains_launch_fill32((uint32_t*)S.contacts, 0u, 2 * (size_t)max_len, st)))
+        return fail(ELP_E_HIP, "fill kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(S.l2i, label2id, sizeof(int32_t) << 18, hipMemcpyHostToDevice, st));
+    rc = elp::chains_launch_heads((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, S.starts, (uint32_t)n_pol, S.stat, st);
+    if (rc) return fail(ELP_E_HIP, "head kernel: %s", hipGetErrorString((hipError_t)rc));
+    uint32_t stat[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(stat, S.stat, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (stat[0] != (uint32_t)n_pol) return fail(ELP_E_STATE, "found %u ring heads, expected %d", stat[0], n_pol);
+    rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, S.starts, stat[0], S.l2i, n_pol,
+                                 max_len, S.bonds, S.lens, S.start3, S.line_len, S.rg2, S.nsl, S.stat + 1, st);
+    if (rc) return fail(ELP_E_HIP, "walk kernel: %s", hipGetErrorString((hipError_t)rc));
+    rc = elp::chains_launch_check_lens(S.lens, n_pol, S.stat + 2, st);
+    if (rc) return fail(ELP_E_HIP, "length check kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(stat, S.stat, sizeof stat, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (stat[1] & 1u) return fail(ELP_E_STATE, "a chain label is not a known polymer id");
+    if (stat[1] & 2u) return fail(ELP_E_STATE, "a chain is longer than %d", max_len);
+    if (stat[3]) return fail(ELP_E_STATE, "%u polymer ids found nowhere", stat[3]);
+    const int n_max = (int)stat[2];
+    if (n_max > elp::kSummaryMaxLen)
+        return fail(ELP_E_ARG, "frame summary supports rings of up to %d monomers (found %d)", elp::kSummaryMaxLen,
+                    n_max);
+    rc = elp::chains_launch_summary(S.bonds, S.lens, n_pol, max_len, n_max, S.sum3, S.mid3, S.contacts, st);
+    if (rc) return fail(ELP_E_HIP, "summary kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(start3, S.start3, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sum3, S.sum3, sizeof(int64_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mid3, S.mid3, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    if (rg2) HIPCHK(hipMemcpyAsync(rg2, S.rg2, sizeof(double) * n_pol, hipMemcpyDeviceToHost, st));
+    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, S.nsl, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, st));
+    if (contacts) HIPCHK(hipMemcpyAsync(contacts, S.contacts, sizeof(uint64_t) * max_len, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return ELP_OK;
 }
 

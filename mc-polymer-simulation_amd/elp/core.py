@@ -103,6 +103,7 @@ def load_elp():
         lib.elp_ring_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
         lib.elp_ring_chain_stats.argtypes = [P, P, I, I, P, P]
         lib.elp_ring_frame_summary.argtypes = [P, P, I, I, P, P, P, P, P, P]
+        lib.elp_ring_frame_summary_noalloc.argtypes = [P, P, I, I, P, P, P, P, P, P]
         lib.elp_lin_chains.argtypes = [P, P, I, I, P, P, P]
         lib.elp_lin_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
         lib.elp_net_create.argtypes = [C.POINTER(P), I, U32]
@@ -236,18 +237,19 @@ class Engine:
                                              nsl.ctypes.data), self.lib)
         return rg2, nsl
 
-    def ring_frame_summary(self, host):
+    def ring_frame_summary(self, host, noalloc=False):
         """One frame of statistics reduced on the device: per polymer id the start site, the sum of its
         monomer positions and its monomer N/2 relative to the start, Rg^2 and the stored-length bond count;
-        contacts[g] = monomer pairs g apart that touch, summed over rings.  Feed to stats.Series(summary=True)."""
+        contacts[g] = monomer pairs g apart that touch, summed over rings.  Feed to stats.Series(summary=True).
+        noalloc: use elp_ring_frame_summary_noalloc (device buffers from the library's static scratch)."""
         l2i = host.label2id()
         n = host.n_pol
         out = {"start3": np.zeros((n, 3), np.int32), "sum3": np.zeros((n, 3), np.int64),
                "mid3": np.zeros((n, 3), np.int32), "rg2": np.zeros(n), "nsl": np.zeros(n, np.int32),
                "contacts": np.zeros(host.N, np.uint64)}
-        _check(self.lib.elp_ring_frame_summary(self.h, l2i.ctypes.data, n, host.N,
-                                               *(out[k].ctypes.data for k in ("start3", "sum3", "mid3", "rg2", "nsl",
-                                                                              "contacts"))), self.lib)
+        fn = self.lib.elp_ring_frame_summary_noalloc if noalloc else self.lib.elp_ring_frame_summary
+        _check(fn(self.h, l2i.ctypes.data, n, host.N,
+                  *(out[k].ctypes.data for k in ("start3", "sum3", "mid3", "rg2", "nsl", "contacts"))), self.lib)
         return out
 
     def ring_tuv(self, host):
