@@ -13,9 +13,13 @@ counts as one attempt, as in the reference kernel).  roofline.achieved =
 algorithmic HBM bytes per launch (the tile is read and written once:
 2 B/site x L^3) / average launch time from HIP events on the engine's stream.
 cpu_baseline = the reference CPU engine efpol (cpupol/, built from its own
-sources into oracle/_ref) timed on the host cores, one independent process
-per core, on a bounded sample; the byte-exact efpol restatement
-(oracle/bin/efpol_port_ring) when that build is absent.
+sources into oracle/_ref) timed on the host cores (the process's CPU affinity
+mask), one independent process per core, on a bounded sample; the byte-exact
+efpol restatement (oracle/bin/efpol_port_ring) when that build is absent.
+accepted_moves_per_s and move_outcomes come from replaying the timed time
+units with the counting kernels (elp_count_moves); "schedules" reports the
+requested schedule and its companion (dense / the reference's cubic
+sub-lattice on 16^3 tiles) side by side.
 """
 import argparse
 import json
@@ -313,6 +317,76 @@ def bench_learned(args):
     return 0
 
 
+STATICS = {
+    "dense": "reference rates on the dense sub-lattice (four sites per 4^3 cell): the reference's moves; "
+             "stored-length ratio about 1 % above the reference schedule's (the move set is not reversible)",
+    "cubic": "reference rates on the reference's cubic sub-lattice (one site per 4^3 cell): the reference's statics",
+}
+
+
+def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, warmup):
+    """Time `steps` time units of one schedule from lat0 (after `warmup` untimed ones), then replay the same
+    time units with move counting from the same lattice (the counting kernels move identically, which is
+    checked) for the accepted-move rate and the rejection shares."""
+    dense = sublattice == "dense"
+    n_iter = n_iter if n_iter is not None else ((13 if dense else 51) if ring else (16 if dense else 64))
+    eng = elp.Engine("ring" if ring else "linear", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC,
+                     n_iter)
+    eng.upload(lat0)
+    host.set_lattice(lat0)
+    host.load_shift(tile)
+    nsite = host.L[0] * host.L[1] * host.L[2]
+    attempts_per_step = n_iter * (4 if ring else 2) * (nsite // 64) * (4 if dense else 1)
+    tau = 0
+    if warmup:
+        eng.run(host.origins(tile, warmup), tau0=tau, seed=host.seed)
+        tau += warmup
+    before = eng.download()
+    org = host.origins(tile, steps)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.run(org, tau0=tau, seed=host.seed, sync=False)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms = eng.last_run_ms()
+    after = eng.download()
+    # the lattice must still hold every labelled chain
+    host.set_lattice(after)
+    eng.upload(before)
+    eng.count_moves(True)
+    eng.run(org, tau0=tau, seed=host.seed)
+    c = eng.move_counts()
+    eng.count_moves(False)
+    if not np.array_equal(eng.download(), after):
+        raise RuntimeError("the counting kernels moved differently from the plain ones")
+    info = eng.launch_info()
+    eng.close()
+    a = c["attempts"]
+    launch_s = kernel_ms * 1e-3 / steps
+    return {
+        "tile": list(tile), "sublattice": sublattice, "n_iter": n_iter, "threads": info["threads"],
+        "blocks": info["blocks"], "attempts_per_step": attempts_per_step, "wall_s": wall,
+        "attempts_per_s": attempts_per_step * steps / wall,
+        "accepted_moves_per_s": c["accepted"] / wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "avg_launch_us": launch_s * 1e6,
+        "achieved_GBs": 2 * (1 if ring else 2) * nsite / launch_s / 1e9,
+        "move_outcomes": {"accepted": c["accepted"] / a, "empty_site": c["empty"] / a, "tile_face": c["face"] / a,
+                          "rejected_other": (a - c["accepted"] - c["empty"] - c["face"]) / a},
+    }
+
+
+def schedule_entry(m, headline):
+    return {
+        "headline": headline,
+        "tile": m["tile"], "sublattice": m["sublattice"], "iterations_per_time_unit": m["n_iter"],
+        "statics": STATICS[m["sublattice"]],
+        "attempts_per_s": m["attempts_per_s"], "accepted_moves_per_s": m["accepted_moves_per_s"],
+        "ms_per_step": m["ms_per_step"], "avg_launch_us": m["avg_launch_us"],
+        "roofline_frac": m["achieved_GBs"] / HBM_PEAK_GBS, "move_outcomes": m["move_outcomes"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,49 +434,28 @@ def main():
         return bench_batch(args, tile)
     if args.workload == "learned":
         return bench_learned(args)
-    dense = args.sublattice == "dense"
     ring = args.chain == "ring"
-    substeps = 4 if ring else 2
-    n_iter = args.n_iter if args.n_iter is not None else ((13 if dense else 51) if ring else (16 if dense else 64))
     N = args.N or (1000 if ring else 500)
     host = (elp.RingHost if ring else elp.LinHost)(N=N, L=args.L, density=args.density, seed=0)
-    eng = elp.Engine("ring" if ring else "linear", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC,
-                     n_iter)
-    eng.upload(host.lattice())
-    host.load_shift(tile)
+    lat0 = host.lattice()
+    head = measure_schedule(elp, host, lat0, ring, tile, args.sublattice, args.n_iter, args.steps, args.warmup)
+    # the companion schedule: the reference's cubic sub-lattice on 16^3 tiles (its statics) beside the
+    # dense one (fastest; stored-length ratio about 1 % higher under reference rates), or the reverse
+    other_sub = "cubic" if args.sublattice == "dense" else "dense"
+    other = measure_schedule(elp, host, lat0, ring, (16, 16, 16), other_sub, None, args.steps, args.warmup)
+
     nsite = host.L[0] * host.L[1] * host.L[2]
     site_bytes = 1 if ring else 2
-    attempts_per_step = n_iter * substeps * (nsite // 64) * (4 if dense else 1)
-
-    tau = 0
-    if args.warmup:
-        org = host.origins(tile, args.warmup)
-        eng.run(org, tau0=tau, seed=host.seed)
-        tau += args.warmup
-    org = host.origins(tile, args.steps)
-    eng.sync()
-    t0 = time.perf_counter()
-    eng.run(org, tau0=tau, seed=host.seed, sync=False)
-    eng.sync()
-    t1 = time.perf_counter()
-    kernel_ms = eng.last_run_ms()
-    wall = t1 - t0
-    # the lattice must still hold every labelled ring
-    host.set_lattice(eng.download())
-
-    value = attempts_per_step * args.steps / wall
-    launch_s = kernel_ms * 1e-3 / args.steps
     bytes_per_launch = 2 * site_bytes * nsite
-    achieved = bytes_per_launch / launch_s / 1e9
-    info = eng.launch_info()
-
     copy_ms = elp.bench_copy(bytes_per_launch // 2, 200)
     copy_gbs = bytes_per_launch / (copy_ms * 1e-3) / 1e9
     kernel = "ring_time_unit2" if ring else "lin_time_unit2"
     traffic = measured_traffic(kernel, args.chain, host.L[0], host.N, tile, args.sublattice)
-    cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+    cores = args.cpu_cores or len(os.sched_getaffinity(0))
     cpu = None if args.no_cpu else cpu_baseline(args.L, N, args.density, cores, warm=300, timed=100, ring=ring)
 
+    wall = head["wall_s"]
+    value = head["attempts_per_s"]
     # GPU attempts cover every site, empty ones included; normalised like efpol's
     # rate, a time unit of the whole melt counts once per monomer
     monomer_rate = host.n_pol * host.N * args.steps / wall
@@ -416,6 +469,8 @@ def main():
         "ms_per_step": wall * 1e3 / args.steps,
         "time_units_per_s": args.steps / wall,
         "monomer_time_units_per_s": monomer_rate,
+        "accepted_moves_per_s": head["accepted_moves_per_s"],
+        "move_outcomes": head["move_outcomes"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -426,26 +481,27 @@ def main():
                         f"({host.n_pol} chains), one time unit per step",
             "tile": list(tile),
             "sublattice": args.sublattice,
-            "iterations_per_time_unit": n_iter,
-            "attempts_per_site_per_time_unit": attempts_per_step / nsite,
-            "threads_per_block": info["threads"],
-            "blocks_per_launch": info["blocks"],
+            "iterations_per_time_unit": head["n_iter"],
+            "attempts_per_site_per_time_unit": head["attempts_per_step"] / nsite,
+            "threads_per_block": head["threads"],
+            "blocks_per_launch": head["blocks"],
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": head["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": "profiles/hbm_traffic.txt (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
                               if traffic else None,
             "kernel": kernel,
             "bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": launch_s * 1e6,
+            "avg_launch_us": head["avg_launch_us"],
             "copy_kernel_same_bytes_GBs": copy_gbs,
         },
+        "schedules": [schedule_entry(head, True), schedule_entry(other, False)],
         "cpu_baseline": cpu,
     }
     if cpu:

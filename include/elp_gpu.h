@@ -100,6 +100,17 @@ int elp_last_run_ms(elp_engine* e, float* ms);
 /* Kernel launch geometry chosen for this engine (for reports). */
 int elp_launch_info(elp_engine* e, int* threads_per_block, int* blocks_per_time_unit, int* lds_bytes);
 
+/* Move counting (no reference counterpart).  With enable = 1 the following
+   elp_run calls launch the counting build of the kernel, which moves exactly
+   as the plain one and classifies every attempt: at an empty site; leaving
+   the tile (the move's sites cross a tile face after its random gates
+   passed); accepted; or rejected otherwise.  enable = 1 also clears the
+   counts; 0 returns to the plain kernel.  elp_move_counts gives, since the
+   last enable: counts[0] attempts, [1] at empty sites, [2] leaving the tile,
+   [3] accepted (it waits for the engine's stream). */
+int elp_count_moves(elp_engine* e, int enable);
+int elp_move_counts(elp_engine* e, uint64_t counts[4]);
+
 /* Threads per workgroup (each thread runs tile slots / threads slots of a
    substep); ELP_E_ARG if that launch shape is not compiled in.  Results do
    not depend on it. */
@@ -161,6 +172,9 @@ int elp_batch_last_run_ms(elp_batch* b, float* ms);
 int elp_batch_info(elp_batch* b, int* threads_per_block, int* blocks_per_time_unit, uint64_t* sites);
 int elp_batch_set_threads(elp_batch* b, int threads);
 int elp_batch_set_rules(elp_batch* b, int rules);
+/* Move counting over all replicas, as elp_count_moves / elp_move_counts. */
+int elp_batch_count_moves(elp_batch* b, int enable);
+int elp_batch_move_counts(elp_batch* b, uint64_t counts[4]);
 
 /*
  * Schedule autotuner (no reference counterpart: the reference fixes its
@@ -176,17 +190,24 @@ typedef struct elp_schedule {
     int threads;
     int units_per_graph;
     double attempts_per_s;   /* measured by the tuner */
+    int rules;               /* ELP_RULES_* the schedule was tuned for */
+    double accepted_per_s;   /* accepted moves per second, measured by the tuner */
 } elp_schedule;
 
-/* Time every compiled batched configuration whose tile divides all boxes
-   on scratch copies of lattices[r] (kind 0 ring, 1 linear; time_units timed
-   per candidate), then the time units per graph for the winner.  The
-   lattices are not modified.  log (optional) receives one line per
-   candidate. */
+/* Tune a batched schedule on scratch copies of lattices[r] (kind 0 ring, 1
+   linear; the lattices are not modified): every compiled configuration whose
+   tile divides all boxes is timed over time_units time units and run again
+   with move counting, and ranked by accepted moves per second; then the time
+   units per graph launch of the winner are timed.  Candidates must sample the
+   same statics: with ELP_RULES_REFERENCE (not reversible, so the stationary
+   law depends on the sub-lattice) only the given sublattice competes; with
+   ELP_RULES_BALANCED sublattice = -1 lets both compete.  log (optional)
+   receives one line per candidate. */
 int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const void* const* lattices,
-                 int time_units, elp_schedule* best, char* log, uint64_t log_len);
+                 int time_units, int rules, int sublattice, elp_schedule* best, char* log, uint64_t log_len);
 /* One-line text record of a schedule, and its parser (round trip exact
-   except attempts_per_s, printed with 7 significant digits). */
+   except the two rates, printed with 7 significant digits; records without
+   rules / accepted_per_s parse as reference rates, 0). */
 int elp_schedule_format(const elp_schedule* s, char* buf, uint64_t len);
 int elp_schedule_parse(const char* text, elp_schedule* s);
 

@@ -5,11 +5,18 @@
 //
 // Stage 1 times every compiled kernel configuration (tile shape, sub-lattice,
 // threads per workgroup) whose tile divides every box, on scratch copies of
-// the caller's lattices, by HIP events around the kernels.  Stage 2 times
-// the time units captured per HIP graph for the winner, by wall clock
-// (host launch cost included).  A configuration is ranked by move attempts
-// per second; the iterations per time unit follow the sub-lattice so that
-// every candidate makes about the same attempts per site per time unit.
+// the caller's lattices, by HIP events around the kernels, then runs it as
+// long again with move counting.  A configuration is ranked by accepted
+// moves per second (attempts per second times the accepted fraction), so
+// attempts at empty sites and moves rejected at tile faces, which thinner
+// tiles make more of, count for nothing.  Only configurations with the same
+// statics compete: under the reference rates the stationary law depends on
+// the sub-lattice (the move set is not reversible), so the caller fixes it;
+// under balanced rates every sub-lattice samples the uniform measure.  The
+// iterations per time unit follow the sub-lattice so that every candidate
+// makes about the same attempts per site per time unit.  Stage 2 times the
+// time units captured per HIP graph for the winner, by wall clock (host
+// launch cost included).
 // The tuned schedule is returned as a struct and as a one-line text record
 // that elp_schedule_parse reads back, so a run can be replayed exactly.
 #include <chrono>
@@ -80,30 +87,48 @@ extern "C" {
 
 int elp_schedule_format(const elp_schedule* s, char* buf, uint64_t len) {
     if (!s || !buf || len == 0) return ELP_E_ARG;
-    int n = snprintf(buf, len, "tile=%dx%dx%d sublattice=%s n_iter=%d threads=%d units_per_graph=%d attempts_per_s=%.6e",
+    int n = snprintf(buf, len,
+                     "tile=%dx%dx%d sublattice=%s n_iter=%d threads=%d units_per_graph=%d attempts_per_s=%.6e "
+                     "rules=%s accepted_per_s=%.6e",
                      s->tt, s->tu, s->tv, s->sublattice == ELP_SUBLAT_DENSE ? "dense" : "cubic", s->n_iter, s->threads,
-                     s->units_per_graph, s->attempts_per_s);
+                     s->units_per_graph, s->attempts_per_s, s->rules == ELP_RULES_BALANCED ? "balanced" : "reference",
+                     s->accepted_per_s);
     return (n > 0 && (uint64_t)n < len) ? ELP_OK : ELP_E_ARG;
 }
 
 int elp_schedule_parse(const char* text, elp_schedule* s) {
     if (!text || !s) return ELP_E_ARG;
-    char sub[16] = {0};
+    char sub[16] = {0}, rules[16] = {0};
     elp_schedule r{};
-    int n = sscanf(text, " tile=%dx%dx%d sublattice=%15s n_iter=%d threads=%d units_per_graph=%d attempts_per_s=%lf",
-                   &r.tt, &r.tu, &r.tv, sub, &r.n_iter, &r.threads, &r.units_per_graph, &r.attempts_per_s);
+    int used = 0;
+    int n = sscanf(text, " tile=%dx%dx%d sublattice=%15s n_iter=%d threads=%d units_per_graph=%d%n", &r.tt, &r.tu,
+                   &r.tv, sub, &r.n_iter, &r.threads, &r.units_per_graph, &used);
     if (n < 7) return ELP_E_ARG;
     if (!strcmp(sub, "dense")) r.sublattice = ELP_SUBLAT_DENSE;
     else if (!strcmp(sub, "cubic")) r.sublattice = ELP_SUBLAT_CUBIC;
     else return ELP_E_ARG;
     if (r.tt <= 0 || r.tu <= 0 || r.tv <= 0 || r.n_iter < 0 || r.threads <= 0 || r.units_per_graph < 0) return ELP_E_ARG;
+    // optional fields, in order
+    const char* rest = text + used;
+    int k = 0;
+    if (sscanf(rest, " attempts_per_s=%lf%n", &r.attempts_per_s, &k) == 1) rest += k;
+    if (sscanf(rest, " rules=%15s%n", rules, &k) == 1) {
+        rest += k;
+        if (!strcmp(rules, "balanced")) r.rules = ELP_RULES_BALANCED;
+        else if (strcmp(rules, "reference")) return ELP_E_ARG;
+    }
+    if (sscanf(rest, " accepted_per_s=%lf", &r.accepted_per_s) == 1) {}
     *s = r;
     return ELP_OK;
 }
 
 int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const void* const* lattices, int time_units,
-                 elp_schedule* best, char* log, uint64_t log_len) {
+                 int rules, int sublattice, elp_schedule* best, char* log, uint64_t log_len) {
     if (!best || !boxes || !lattices || n_rep <= 0 || time_units <= 0 || (kind != 0 && kind != 1)) return ELP_E_ARG;
+    if (rules != ELP_RULES_REFERENCE && rules != ELP_RULES_BALANCED) return ELP_E_ARG;
+    if (sublattice != ELP_SUBLAT_CUBIC && sublattice != ELP_SUBLAT_DENSE && !(sublattice == -1 && rules == ELP_RULES_BALANCED))
+        return ELP_E_ARG;   // reference rates: the sub-lattice fixes the statics
+    if (kind == 1 && rules == ELP_RULES_BALANCED) return ELP_E_ARG;   // balanced rates exist for rings
     Log lg{log, log_len};
     if (log && log_len) log[0] = 0;
     const size_t bps = kind == 0 ? 1 : 2;
@@ -122,6 +147,7 @@ int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const vo
                            : elp_lin_batch_create(&g.b, device, n_rep, boxes, c.tt, c.tu, c.tv, c.sublattice, n_iter);
         if (rc) return rc;
         if ((rc = elp_batch_set_threads(g.b, c.threads))) return rc;
+        if (rules && (rc = elp_batch_set_rules(g.b, rules))) return rc;
         for (int r = 0; r < n_rep; r++) {
             const uint64_t nb = (uint64_t)boxes[3 * r] * boxes[3 * r + 1] * boxes[3 * r + 2] * bps;
             if ((rc = elp_batch_upload(g.b, r, lattices[r], nb))) return rc;
@@ -130,7 +156,9 @@ int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const vo
     };
 
     // stage 1: kernel configuration
+    double win_frac = 0;   // accepted fraction of the winner
     for (const Cand& c : kCands) {
+        if (sublattice != -1 && c.sublattice != sublattice) continue;
         bool divides = true;
         for (int r = 0; r < n_rep && divides; r++)
             divides = boxes[3 * r] % c.tt == 0 && boxes[3 * r + 1] % c.tu == 0 && boxes[3 * r + 2] % c.tv == 0;
@@ -148,10 +176,22 @@ int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const vo
         float ms = 0;
         if ((rc = elp_batch_last_run_ms(g.b, &ms))) return rc;
         const double aps = attempts_per_unit(kind, c.sublattice, n_iter, sites) * time_units / (ms * 1e-3);
-        lg.add("kernel tile=%dx%dx%d sublattice=%s threads=%d n_iter=%d: %.3f ms/unit %.4e attempts/s\n", c.tt, c.tu,
-               c.tv, c.sublattice == ELP_SUBLAT_DENSE ? "dense" : "cubic", c.threads, n_iter, ms / time_units, aps);
-        if (!found || aps > win.attempts_per_s) {
-            win = elp_schedule{c.tt, c.tu, c.tv, c.sublattice, n_iter, c.threads, 0, aps};
+        // the same number of time units again, counting what the moves do
+        uint64_t cnt[4] = {0, 0, 0, 0};
+        if ((rc = elp_batch_count_moves(g.b, 1)) ||
+            (rc = elp_batch_run(g.b, time_units, org.data(), tau0.data(), keys.data(), 0)) ||
+            (rc = elp_batch_move_counts(g.b, cnt)) || (rc = elp_batch_count_moves(g.b, 0)))
+            return rc;
+        const double frac = cnt[0] ? (double)cnt[3] / (double)cnt[0] : 0.0;
+        const double acc = aps * frac;
+        lg.add("kernel tile=%dx%dx%d sublattice=%s threads=%d n_iter=%d: %.3f ms/unit %.4e attempts/s, accepted %.4f "
+               "(empty %.4f, tile face %.4f): %.4e accepted/s\n",
+               c.tt, c.tu, c.tv, c.sublattice == ELP_SUBLAT_DENSE ? "dense" : "cubic", c.threads, n_iter,
+               ms / time_units, aps, frac, cnt[0] ? (double)cnt[1] / cnt[0] : 0.0,
+               cnt[0] ? (double)cnt[2] / cnt[0] : 0.0, acc);
+        if (!found || acc > win.accepted_per_s) {
+            win = elp_schedule{c.tt, c.tu, c.tv, c.sublattice, n_iter, c.threads, 0, aps, rules, acc};
+            win_frac = frac;
             found = true;
         }
     }
@@ -174,11 +214,13 @@ int elp_autotune(int device, int kind, int n_rep, const int32_t* boxes, const vo
             return rc;
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const double aps = attempts_per_unit(kind, win.sublattice, win.n_iter, sites) * units / wall;
-        lg.add("units_per_graph=%d: %.3f ms/unit wall %.4e attempts/s\n", G, wall * 1e3 / units, aps);
+        lg.add("units_per_graph=%d: %.3f ms/unit wall %.4e attempts/s %.4e accepted/s\n", G, wall * 1e3 / units, aps,
+               aps * win_frac);
         if (best_wall == 0 || wall < best_wall) {
             best_wall = wall;
             win.units_per_graph = G;
             win.attempts_per_s = aps;
+            win.accepted_per_s = aps * win_frac;
         }
     }
     *best = win;

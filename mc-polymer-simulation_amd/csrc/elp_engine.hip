@@ -76,7 +76,20 @@ struct elp_engine {
     bool timed;
     uint32_t trans[4];
     uint32_t rules;        // ELP_RULES_*
+    // move counting (elp_count_moves): device counters of attempts at empty
+    // sites, leaving the tile and accepted; attempts counted on the host
+    unsigned long long* d_counts = nullptr;
+    bool counting = false;
+    uint64_t attempts = 0;
 };
+
+// move attempts of one time unit: every slot of every tile in every substep
+static uint64_t attempts_per_unit(int kind, int LT, int LU, int LV, int tt, int tu, int tv, int sublattice,
+                                  int n_iter) {
+    const uint64_t ntile = (uint64_t)(LT / tt) * (LU / tu) * (LV / tv);
+    const uint64_t nslot = (uint64_t)(tt / 4) * (tu / 4) * (tv / 4) * (sublattice == ELP_SUBLAT_DENSE ? 4 : 1);
+    return ntile * nslot * (uint64_t)n_iter * (kind == 0 ? 4u : 2u);
+}
 
 static int launch_one(elp_engine* e, bool probe, const elp::RingLaunch& p) {
     const int nblocks = (e->LT / e->tt) * (e->LU / e->tu) * (e->LV / e->tv);
@@ -171,6 +184,7 @@ int elp_destroy(elp_engine* e) {
     (void)hipEventDestroy(e->ev1);
     (void)hipStreamDestroy(e->stream);
     (void)hipFree(e->d_lat);
+    if (e->d_counts) (void)hipFree(e->d_counts);
     delete e;
     return ELP_OK;
 }
@@ -211,7 +225,11 @@ int elp_run(elp_engine* e, int n_time, const int32_t* origins, uint64_t tau0, ui
     p.seed = seed; p.stream = stream;
     p.n_iter = e->n_iter;
     p.rules = e->rules;
+    p.counts = e->counting ? e->d_counts : nullptr;
     for (int i = 0; i < 4; i++) p.trans[i] = e->trans[i];
+    if (e->counting)
+        e->attempts += (uint64_t)n_time * attempts_per_unit(e->kind, e->LT, e->LU, e->LV, e->tt, e->tu, e->tv,
+                                                            e->sublattice, e->n_iter);
     HIPCHK(hipEventRecord(e->ev0, e->stream));
     for (int i = 0; i < n_time; i++) {
         p.ot = origins[3 * i]; p.ou = origins[3 * i + 1]; p.ov = origins[3 * i + 2];
@@ -221,6 +239,30 @@ int elp_run(elp_engine* e, int n_time, const int32_t* origins, uint64_t tau0, ui
     }
     HIPCHK(hipEventRecord(e->ev1, e->stream));
     e->timed = true;
+    return ELP_OK;
+}
+
+int elp_count_moves(elp_engine* e, int enable) {
+    if (!e) return fail(ELP_E_ARG, "NULL engine");
+    HIPCHK(hipSetDevice(e->device));
+    if (enable) {
+        if (!e->d_counts) HIPCHK(hipMalloc(&e->d_counts, 3 * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(e->d_counts, 0, 3 * sizeof(unsigned long long), e->stream));
+        e->attempts = 0;
+    }
+    e->counting = enable != 0;
+    return ELP_OK;
+}
+
+int elp_move_counts(elp_engine* e, uint64_t counts[4]) {
+    if (!e || !counts) return fail(ELP_E_ARG, "NULL argument");
+    if (!e->d_counts) return fail(ELP_E_STATE, "move counting was never enabled");
+    HIPCHK(hipSetDevice(e->device));
+    unsigned long long c[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c, e->d_counts, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    counts[0] = e->attempts;
+    counts[1] = c[0]; counts[2] = c[1]; counts[3] = c[2];
     return ELP_OK;
 }
 
@@ -283,10 +325,14 @@ struct elp_batch {
     int g_units = 0;
     const void* g_steps = nullptr;
     uint32_t rules = 0;                    // ELP_RULES_*
+    unsigned long long* d_counts = nullptr;   // move counting, as in elp_engine
+    bool counting = false;
+    uint64_t attempts = 0;
 };
 
 static int batch_launch_one(elp_batch* b, bool probe, int k) {
-    elp::BatchLaunch bl{b->d_reps, b->d_rep_of_tile, b->d_steps, b->d_cursor, k, b->n_rep, b->n_iter, b->rules};
+    elp::BatchLaunch bl{b->d_reps, b->d_rep_of_tile, b->d_steps, b->d_cursor, k, b->n_rep, b->n_iter, b->rules,
+                        b->counting ? b->d_counts : nullptr};
     const bool dense = b->sublattice == ELP_SUBLAT_DENSE;
     if (b->kind == 0)
         return elp::ring_batch_launch(b->tt, b->tu, b->tv, dense, b->threads, probe, (int)b->n_tiles, b->stream,
@@ -305,6 +351,7 @@ static void batch_free(elp_batch* b) {
     (void)hipFree(b->d_rep_of_tile);
     (void)hipFree(b->d_steps);
     (void)hipFree(b->d_cursor);
+    if (b->d_counts) (void)hipFree(b->d_counts);
     delete b;
 }
 
@@ -462,6 +509,36 @@ int elp_batch_download(elp_batch* b, int rep, void* lattice, uint64_t nbytes) {
     return ELP_OK;
 }
 
+int elp_batch_count_moves(elp_batch* b, int enable) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (enable) {
+        if (!b->d_counts) HIPCHK(hipMalloc(&b->d_counts, 3 * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(b->d_counts, 0, 3 * sizeof(unsigned long long), b->stream));
+        b->attempts = 0;
+    }
+    if ((enable != 0) != b->counting && b->gexec) {   // the cached graph holds the other kernel
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+        b->g_units = 0;
+    }
+    b->counting = enable != 0;
+    return ELP_OK;
+}
+
+int elp_batch_move_counts(elp_batch* b, uint64_t counts[4]) {
+    if (!b || !counts) return fail(ELP_E_ARG, "NULL argument");
+    if (!b->d_counts) return fail(ELP_E_STATE, "move counting was never enabled");
+    HIPCHK(hipSetDevice(b->device));
+    unsigned long long c[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c, b->d_counts, sizeof c, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    counts[0] = b->attempts;
+    counts[1] = c[0]; counts[2] = c[1]; counts[3] = c[2];
+    return ELP_OK;
+}
+
 int elp_batch_run(elp_batch* b, int n_time, const int32_t* origins, const uint64_t* tau0, const uint32_t* keys,
                   int units_per_graph) {
     if (!b) return fail(ELP_E_ARG, "NULL batch");
@@ -502,6 +579,9 @@ int elp_batch_run(elp_batch* b, int n_time, const int32_t* origins, const uint64
     HIPCHK(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(elp::BatchRep) * R, hipMemcpyHostToDevice, b->stream));
     hipLaunchKernelGGL(elp_set_cursor, dim3(1), dim3(1), 0, b->stream, b->d_cursor, 0);
     HIPCHK(hipGetLastError());
+    if (b->counting)
+        b->attempts += (uint64_t)n_time * b->n_tiles *
+                       attempts_per_unit(b->kind, b->tt, b->tu, b->tv, b->tt, b->tu, b->tv, b->sublattice, b->n_iter);
     const int G = units_per_graph;
     const int replays = G > 0 ? n_time / G : 0;
     if (replays > 0) {

@@ -16,6 +16,7 @@ struct RingLaunch {
     int n_iter;              // iterations of the 4-substep sequence
     uint32_t trans[4];       // first bond of each decomposition (SetGPUTrans)
     uint32_t rules;          // ring move rates: 0 reference, 1 balanced (ELP_RULES_*)
+    unsigned long long* counts;   // counting launches: attempts at empty sites, leaving the tile, accepted
 };
 
 using LinLaunch = RingLaunch;
@@ -48,11 +49,19 @@ struct BatchLaunch {
     int n_rep;
     int n_iter;
     uint32_t rules;
+    unsigned long long* counts;   // counting launches (see RingLaunch)
 };
 
+// Outcome of one move attempt, returned by the move functions: the site is
+// empty; the move would touch a site outside the tile (after its random
+// gates); the move is rejected otherwise; the move is made.
+enum : uint32_t { kMoveEmpty = 0, kMoveFace = 1, kMoveRejected = 2, kMoveAccepted = 3 };
+
 // Launch one time unit (nblocks = number of tiles).  probe=true only checks
-// that the schedule is compiled in.  Returns 0, a hipError_t, or -1 when the
-// schedule is not available.
+// that the schedule is compiled in.  A non-null p.counts launches the
+// counting instantiation of the kernel, which adds the outcomes of its
+// attempts to counts[0..2] (empty, face, accepted) and moves exactly as the
+// plain one.  Returns 0, a hipError_t, or -1 when the schedule is not available.
 int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                 uint8_t* glat, const RingLaunch& p);
 int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,

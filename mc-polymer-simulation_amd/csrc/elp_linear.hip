@@ -252,8 +252,10 @@ __global__ __launch_bounds__(THREADS) void lin_time_unit(uint16_t* __restrict__ 
 // site word, then one accept/reject decision per case.  Same per-site
 // results as the functions above.
 // =====================================================================
+// The move functions return their outcome (kMoveEmpty ... kMoveAccepted) for
+// counting launches.
 template <class G>
-__device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
+__device__ __forceinline__ uint32_t lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
     const uint32_t a = G::addr(P);
     uint32_t w = lat[a];
     const uint32_t next = w & 0xfu;
@@ -272,6 +274,7 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
         lat[a] = (uint16_t)(toX ? wX : wW);
         lat[na] = (uint16_t)(toX ? xX : xW);
     }
+    return (w & 0xffu) == 0u ? kMoveEmpty : ((next != 0u) & G::out(Pn)) ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
 }
 
 // The two sites a case reads are X2 = P + delta(c2) and X1 = base + delta(c1):
@@ -285,8 +288,8 @@ __device__ __forceinline__ void lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
 // holds fold_pair in bits 24-31 above the packed offset of its first bond
 // (nPrev) in bits 0-23, so the split's X1 is one add of that word.
 template <class G>
-__device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w,
-                                                    uint32_t wp) {
+__device__ __forceinline__ uint32_t lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w,
+                                                        uint32_t wp) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     // stored-length cases: split the chosen bond, or grow the end
@@ -353,11 +356,17 @@ __device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, u
     const uint32_t x2f = pbF ? x2 : x2fnext;
     const uint32_t x1F = (x1f & ~0xfu) | bond, x2F = (x2f & ~0xf0u) | (bond << 4);
     const uint32_t wF = w & ~(kL0 | 0xffu);
-    if (occ & (okGrow | okSplit | okEnd | okFold)) {
+    const bool acc = occ & (okGrow | okSplit | okEnd | okFold);
+    if (acc) {
         lat[a] = (uint16_t)(okGrow ? wG : okSplit ? wS : okEnd ? wE : wF);
         lat[a1] = (uint16_t)(okGrow ? x1G : okSplit ? x1S : okEnd ? x1E : x1F);
         if (okSplit | okFold) lat[a2] = (uint16_t)(okSplit ? x2S : x2F);
     }
+    // a face rejection: the random gates of the case pass (growth: a valid
+    // new bond; without stored length: bit 4, and bits 5-7 for an end) and
+    // the case's sites leave the tile
+    const bool pass = sl ? !((bondS == 0u) & nbBad) : (gate & (!end | (((r >> 5) & 7u) == 0u)));
+    return !occ ? kMoveEmpty : (pass & G::out(X1 | X2)) ? kMoveFace : acc ? kMoveAccepted : kMoveRejected;
 }
 
 // TransMove of a wave in which no occupied site is a chain end: every
@@ -366,8 +375,8 @@ __device__ __forceinline__ void lin_trans_move_ends(uint16_t* lat, uint32_t P, u
 // the fold needs no end test.  Same outcome as lin_trans_move_ends for such a
 // wave; empty lanes read in-tile sites and write nothing.
 template <class G>
-__device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a,
-                                                        uint32_t w, uint32_t wp) {
+__device__ __forceinline__ uint32_t lin_trans_move_interior(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a,
+                                                            uint32_t w, uint32_t wp) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     const uint32_t pbS = (r >> 2) & 1u;
@@ -407,11 +416,13 @@ __device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t 
     const uint32_t x1F = (x1f & ~0xfu) | bond, x2F = (x2f & ~0xf0u) | (bond << 4);
     const uint32_t wF = w & ~(kL0 | 0xffu);
     // okSplit implies sl and okFold !sl, so sl picks the outcome
-    if (occ & (okSplit | okFold)) {
+    const bool acc = occ & (okSplit | okFold);
+    if (acc) {
         lat[a] = (uint16_t)(sl ? wS : wF);
         lat[a1] = (uint16_t)(sl ? x1S : x1F);
         lat[a2] = (uint16_t)(sl ? x2S : x2F);
     }
+    return !occ ? kMoveEmpty : ((sl | gate) & !inTile) ? kMoveFace : acc ? kMoveAccepted : kMoveRejected;
 }
 
 // Growing and retracting need a chain end, 2 of every N monomers: a wave with
@@ -419,14 +430,14 @@ __device__ __forceinline__ void lin_trans_move_interior(uint16_t* lat, uint32_t 
 // Both forms read the table word of the chosen bond (bondS, r & 3), which is
 // read before the branch so that the read and the test overlap.
 template <class G>
-__device__ __forceinline__ void lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
+__device__ __forceinline__ uint32_t lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     const uint32_t w = lat[a];
     const uint32_t bondS = ((r >> 2) & 1u) ? (w >> 4) & 0xfu : w & 0xfu;
     const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
     const bool occEnd = ((w & 0xffu) != 0u) & (((w & 0xfu) == 0u) | ((w & 0xf0u) == 0u));
-    if (__ballot(occEnd) != 0ull) lin_trans_move_ends<G>(lat, P, r, a, w, wp);
-    else lin_trans_move_interior<G>(lat, P, r, a, w, wp);
+    if (__ballot(occEnd) != 0ull) return lin_trans_move_ends<G>(lat, P, r, a, w, wp);
+    return lin_trans_move_interior<G>(lat, P, r, a, w, wp);
 }
 
 // linpol's substep sequence {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490)
@@ -443,23 +454,24 @@ struct LinMoves {
         }
     }
     template <class G>
-    __device__ static __forceinline__ void move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {
-        if (s == 0) lin_diffuse_sl2<G>(lat, P);
-        else lin_trans_move2<G>(lat, P, r);
+    __device__ static __forceinline__ uint32_t move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {
+        if (s == 0) return lin_diffuse_sl2<G>(lat, P);
+        return lin_trans_move2<G>(lat, P, r);
     }
 };
 
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool COUNT = false>
 __global__ __launch_bounds__(THREADS) void lin_time_unit2(uint16_t* __restrict__ glat, LinLaunch p) {
     extern __shared__ uint16_t tile[];
-    tile_unit<LinMoves, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p), xcd_tile(blockIdx.x, gridDim.x));
+    tile_unit<LinMoves, TT, TU, TV, DENSE, THREADS, PAD, COUNT>(glat, tile, unit_args(p),
+                                                                xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // One time unit of every replica of a batch (one workgroup per tile).
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool COUNT = false>
 __global__ __launch_bounds__(THREADS) void lin_batch_unit(uint16_t* __restrict__ gbase, BatchLaunch b) {
     extern __shared__ uint16_t tile[];
-    batch_unit<LinMoves, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
+    batch_unit<LinMoves, TT, TU, TV, DENSE, THREADS, PAD, COUNT>(gbase, tile, b);
 }
 
 // variant 1: lin_time_unit (cell-major LDS); 2: lin_time_unit2 compact;
@@ -478,6 +490,12 @@ static int launch_lds(K kernel, size_t lds, int nblocks, int threads, hipStream_
 #define ELP_LIN_CASE(TT, TU, TV, DENSE, TH)                                                                   \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                                 \
         if (probe) return 0;                                                                                   \
+        if (p.counts) {                                                                                        \
+            constexpr int PADC = (TT == 16 && TU == 16 && TV == 16) ? 1 : 0;                                   \
+            return launch_lds(lin_time_unit2<TT, TU, TV, DENSE, TH, PADC, true>,                               \
+                              sizeof(uint16_t) * PTile<TT, TU, TV, PADC>::NSITE + LinMoves::EXTRA, nblocks, TH, \
+                              stream, glat, p);                                                                \
+        }                                                                                                      \
         if (variant == 1)                                                                                      \
             return launch_lds(lin_time_unit<TT, TU, TV, DENSE, TH>, sizeof(uint16_t) * TT * TU * TV, nblocks, TH, \
                               stream, glat, p);                                                                \
@@ -517,7 +535,8 @@ int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int 
         if (probe) return 0;                                                                           \
         constexpr int PAD = (TT == 16 && TU == 16 && TV == 16) ? 1 : 0;                                \
         constexpr size_t lds = sizeof(uint16_t) * PTile<TT, TU, TV, PAD>::NSITE + LinMoves::EXTRA;     \
-        auto kernel = lin_batch_unit<TT, TU, TV, DENSE, TH, PAD>;                                      \
+        auto kernel = b.counts ? lin_batch_unit<TT, TU, TV, DENSE, TH, PAD, true>                      \
+                               : lin_batch_unit<TT, TU, TV, DENSE, TH, PAD>;                           \
         if (lds > 65536) {                                                                             \
             hipError_t e = hipFuncSetAttribute((const void*)kernel,                                    \
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \

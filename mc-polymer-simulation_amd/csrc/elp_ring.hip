@@ -282,8 +282,9 @@ __device__ __forceinline__ bool forw_move_first(uint32_t s, uint32_t ns, uint32_
 // still waits for three dependent LDS reads.  An empty site needs no test of
 // its own: next = 0 puts the partner on the site itself, and neither end
 // then holds stored length.
+// Returns the outcome (kMoveEmpty ... kMoveAccepted) for counting launches.
 template <class G, bool PAIRS = false>
-__device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
+__device__ __forceinline__ uint32_t trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -318,6 +319,7 @@ __device__ __forceinline__ void trans_forw2(uint8_t* lat, uint32_t P, uint32_t r
         lat[da] = (uint8_t)d1;
         if (!moveFirst) lat[na] = (uint8_t)(ns & 0x1fu);
     }
+    return s == 0u ? kMoveEmpty : !inTile ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
 }
 
 // rules = 1 (balanced): a fold forced by stored length at exactly one end
@@ -381,7 +383,7 @@ __device__ __forceinline__ uint32_t diffuse_partner(const DiffuseParts& d, uint3
 }
 
 template <class G, bool TABLE = false>
-__device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
+__device__ __forceinline__ uint32_t trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -409,10 +411,11 @@ __device__ __forceinline__ void trans_back2(uint8_t* lat, uint32_t P, uint32_t r
         lat[a] = (uint8_t)s1;
         lat[wa] = (uint8_t)w1;
     }
+    return s == 0u ? kMoveEmpty : G::out(Ps | Pw) ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
 }
 
 template <class G>
-__device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
+__device__ __forceinline__ uint32_t diffuse_sl2(uint8_t* lat, uint32_t P) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -433,6 +436,7 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
         lat[a] = (uint8_t)s1;
         lat[na] = (uint8_t)n1;
     }
+    return s == 0u ? kMoveEmpty : !inTile ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
 }
 
 // Balanced rates with the LDS tables: the three moves a slot may draw
@@ -445,7 +449,7 @@ __device__ __forceinline__ void diffuse_sl2(uint8_t* lat, uint32_t P) {
 // beside the combine code.  The outcome and the writes are those of
 // trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
 template <class G>
-__device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
+__device__ __forceinline__ uint32_t balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
     const uint2* tab = ring_tab<G>(lat);
     const uint32_t ty = r >> 6;
     const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
@@ -487,6 +491,8 @@ __device__ __forceinline__ void balanced_move(uint8_t* lat, uint32_t P, uint32_t
         if (okDSL | okTB | (okTF & !mfTF)) lat[na] = (uint8_t)(okTF ? (ns & 0x1fu) : okDSL ? nDSL : 0u);
         if (okTF | okTB) lat[xa] = (uint8_t)(okTF ? xTF : xTB);
     }
+    const bool face = isDSL ? G::out(Pn) : G::out(Pn | X);
+    return s == 0u ? kMoveEmpty : face ? kMoveFace : (okTF | okDSL | okTB) ? kMoveAccepted : kMoveRejected;
 }
 
 // ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
@@ -508,41 +514,40 @@ struct RingMovesT {
         }
     }
     template <class G>
-    __device__ static __forceinline__ void move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
+    __device__ static __forceinline__ uint32_t move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
         if (rules) {
             // balanced rates: every slot draws its move from bits 6-7 of its
             // byte (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the
             // proportions of the reference sequence), so each substep is a
             // symmetric kernel and preserves the uniform measure by itself
             if constexpr (PAIRS) {
-                balanced_move<G>(lat, P, r);
+                return balanced_move<G>(lat, P, r);
             } else {
                 const uint32_t ty = r >> 6;
-                if (ty < 2u) trans_forw2<G, PAIRS>(lat, P, r);
-                else if (ty == 2u) diffuse_sl2<G>(lat, P);
-                else trans_back2<G, PAIRS>(lat, P, r, rules);
+                if (ty < 2u) return trans_forw2<G, PAIRS>(lat, P, r);
+                if (ty == 2u) return diffuse_sl2<G>(lat, P);
+                return trans_back2<G, PAIRS>(lat, P, r, rules);
             }
-            return;
         }
-        if (s == 1) diffuse_sl2<G>(lat, P);
-        else if (s == 3) trans_back2<G, PAIRS>(lat, P, r, rules);
-        else trans_forw2<G, PAIRS>(lat, P, r);
+        if (s == 1) return diffuse_sl2<G>(lat, P);
+        if (s == 3) return trans_back2<G, PAIRS>(lat, P, r, rules);
+        return trans_forw2<G, PAIRS>(lat, P, r);
     }
 };
 using RingMoves = RingMovesT<false>;
 
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false, bool COUNT = false>
 __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p) {
     __shared__ __align__(16) uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
-    tile_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, unit_args(p),
-                                                                  xcd_tile(blockIdx.x, gridDim.x));
+    tile_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD, COUNT>(glat, tile, unit_args(p),
+                                                                         xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // One time unit of every replica of a batch (one workgroup per tile).
-template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false>
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false, bool COUNT = false>
 __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__ gbase, BatchLaunch b) {
     __shared__ __align__(16) uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<PAIRS>::EXTRA];
-    batch_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD>(gbase, tile, b);
+    batch_unit<RingMovesT<PAIRS>, TT, TU, TV, DENSE, THREADS, PAD, COUNT>(gbase, tile, b);
 }
 
 // ring_time_unit2 for variants 2-4: P-indexed (padded) LDS only for
@@ -550,6 +555,12 @@ __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__
 // LDS bond-pair table of TransForw.
 template <int TT, int TU, int TV, bool DENSE, int TH>
 static void launch2(int variant, int nblocks, hipStream_t stream, uint8_t* glat, const RingLaunch& p) {
+    constexpr int PADC = (TT == 16 && TU == 16 && TV <= 16) ? 1 : 0;
+    if (p.counts) {   // counting instantiation of the default formulation (variant 4)
+        hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, PADC, true, true>), dim3(nblocks), dim3(TH), 0,
+                           stream, glat, p);
+        return;
+    }
     if constexpr (TT == 16 && TU == 16 && TV <= 16) {
         if (variant == 4)
             hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, 1, true>), dim3(nblocks), dim3(TH), 0, stream,
@@ -574,7 +585,8 @@ static void launch2(int variant, int nblocks, hipStream_t stream, uint8_t* glat,
 #define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                              \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
-        if (variant == 1 && p.rules) return (int)hipErrorInvalidValue; /* first version: reference rates */ \
+        if (variant == 1 && (p.rules || p.counts))                                                         \
+            return (int)hipErrorInvalidValue; /* first version: reference rates, no counting */           \
         if (variant == 1)                                                                                  \
             hipLaunchKernelGGL((ring_time_unit<TT, TU, TV, DENSE, TH>), dim3(nblocks), dim3(TH), 0, stream, \
                                glat, p);                                                                   \
@@ -613,7 +625,10 @@ int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \
         constexpr int PAD = (TT == 16 && TU == 16 && TV <= 16) ? 1 : 0;                                    \
-        if (batch_pairs)                                                                                   \
+        if (b.counts)                                                                                      \
+            hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD, true, true>), dim3(nblocks),    \
+                               dim3(TH), 0, stream, gbase, b);                                             \
+        else if (batch_pairs)                                                                              \
             hipLaunchKernelGGL((ring_batch_unit<TT, TU, TV, DENSE, TH, PAD, true>), dim3(nblocks), dim3(TH), 0, \
                                stream, gbase, b);                                                          \
         else                                                                                               \

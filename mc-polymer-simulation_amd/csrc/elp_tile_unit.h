@@ -5,7 +5,8 @@
 // by packed guarded coordinates, PTile), runs n_iter iterations of the move
 // set's substeps with a workgroup barrier after each substep, and writes the
 // tile back in place.  The move set M supplies the site type, the number of
-// substeps per iteration and the move of substep s:
+// substeps per iteration and the move of substep s, which returns its
+// outcome (kMoveEmpty ... kMoveAccepted):
 //   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules),
 // and fills any LDS tables it keeps after the tile: M::init<G>(lds, tid).
 //
@@ -29,11 +30,19 @@ struct UnitArgs {
     uint32_t tau, seed, stream;
     int n_iter;
     uint32_t rules;          // move rates (ring: 0 reference, 1 balanced)
+    unsigned long long* counts;   // COUNT: empty, face, accepted
 };
 
 __device__ __forceinline__ UnitArgs unit_args(const RingLaunch& p) {
     return UnitArgs{p.LT, p.LU, p.LV, p.ntile_t, p.ntile_u, p.ot, p.ou, p.ov, p.tau, p.seed, p.stream, p.n_iter,
-                    p.rules};
+                    p.rules, p.counts};
+}
+
+// Sum of v over the wave (butterfly over the 64 lanes).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
 }
 
 // Copy one tile between the box (row-major, periodic) and LDS, in either
@@ -108,7 +117,7 @@ __device__ __forceinline__ void tile_copy(T* __restrict__ glat, T* tile, const U
     }
 }
 
-template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool COUNT = false>
 __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat, typename M::site_t* tile,
                                           const UnitArgs& p, uint32_t tileId) {
     using G = PTile<TT, TU, TV, PAD>;
@@ -145,6 +154,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     }
     M::template init<G>(tile, tid);
     __syncthreads();
+    uint32_t nEmpty = 0, nFace = 0, nAcc = 0;   // COUNT: outcomes of this thread's attempts
 
     for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
         const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
@@ -167,7 +177,12 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                     uint32_t P;
                     if (DENSE) P = ((cu + ju[k]) & (3u << G::SU)) | (vb[k] ^ cv);
                     else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                    M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu, p.rules);
+                    const uint32_t out = M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu, p.rules);
+                    if constexpr (COUNT) {
+                        nEmpty += out == kMoveEmpty;
+                        nFace += out == kMoveFace;
+                        nAcc += out == kMoveAccepted;
+                    }
                 }
                 __syncthreads();
             }
@@ -180,12 +195,23 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     int tidStore = tid;
     asm volatile("" : "+v"(tidStore));
     tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tidStore);
+    if constexpr (COUNT) {
+        // one sum per wave, added by its first lane (vector atomics)
+        nEmpty = wave_sum(nEmpty);
+        nFace = wave_sum(nFace);
+        nAcc = wave_sum(nAcc);
+        if ((tid & 63) == 0) {
+            atomicAdd(&p.counts[0], (unsigned long long)nEmpty);
+            atomicAdd(&p.counts[1], (unsigned long long)nFace);
+            atomicAdd(&p.counts[2], (unsigned long long)nAcc);
+        }
+    }
 }
 
 // Workgroup of a batched launch: global tile g (XCD-aware order over all
 // replicas' tiles) -> replica r = rep_of_tile[g], its local tile g - tile0,
 // and the origin / time unit of replica r for time unit (*cursor + k).
-template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool COUNT = false>
 __device__ __forceinline__ void batch_unit(typename M::site_t* __restrict__ gbase, typename M::site_t* tile,
                                            const BatchLaunch& b) {
     const uint32_t g = xcd_tile(blockIdx.x, gridDim.x);
@@ -193,8 +219,8 @@ __device__ __forceinline__ void batch_unit(typename M::site_t* __restrict__ gbas
     const BatchRep rep = b.reps[r];
     const BatchStep st = b.steps[(size_t)(*b.cursor + b.k) * (size_t)b.n_rep + r];
     const UnitArgs p{rep.LT, rep.LU, rep.LV, rep.ntile_t, rep.ntile_u, st.ot, st.ou, st.ov,
-                     st.tau, rep.seed, rep.stream, b.n_iter, b.rules};
-    tile_unit<M, TT, TU, TV, DENSE, THREADS, PAD>(gbase + rep.site_off, tile, p, g - rep.tile0);
+                     st.tau, rep.seed, rep.stream, b.n_iter, b.rules, b.counts};
+    tile_unit<M, TT, TU, TV, DENSE, THREADS, PAD, COUNT>(gbase + rep.site_off, tile, p, g - rep.tile0);
 }
 
 }  // namespace elp

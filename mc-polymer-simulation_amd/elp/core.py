@@ -27,7 +27,8 @@ _libs = {}
 class Schedule(C.Structure):
     """elp_schedule of include/elp_gpu.h."""
     _fields_ = [("tt", C.c_int), ("tu", C.c_int), ("tv", C.c_int), ("sublattice", C.c_int), ("n_iter", C.c_int),
-                ("threads", C.c_int), ("units_per_graph", C.c_int), ("attempts_per_s", C.c_double)]
+                ("threads", C.c_int), ("units_per_graph", C.c_int), ("attempts_per_s", C.c_double),
+                ("rules", C.c_int), ("accepted_per_s", C.c_double)]
 
     @property
     def tile(self):
@@ -45,9 +46,15 @@ class Schedule(C.Structure):
         return s
 
 
-def autotune(kind, lattices, boxes, time_units=20, device=0):
-    """Fastest compiled schedule for these replicas (elp_autotune); returns
+def autotune(kind, lattices, boxes, time_units=20, device=0, rules=None, sublattice=None):
+    """Compiled schedule with the most accepted moves per second for these
+    replicas, among schedules with the same statics (elp_autotune): under the
+    reference rates (default) only `sublattice` (default dense) competes;
+    under RULES_BALANCED sublattice=None lets both compete.  Returns
     (Schedule, log text).  The lattices are only read."""
+    rules = RULES_REFERENCE if rules is None else rules
+    if sublattice is None:
+        sublattice = SUBLAT_DENSE if rules == RULES_REFERENCE else -1
     lib = load_elp()
     arrs = [np.ascontiguousarray(a, dtype=np.uint8 if kind == "ring" else np.uint16) for a in lattices]
     ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
@@ -55,7 +62,7 @@ def autotune(kind, lattices, boxes, time_units=20, device=0):
     best = Schedule()
     log = C.create_string_buffer(8192)
     _check(lib.elp_autotune(device, 0 if kind == "ring" else 1, len(arrs), bx.ctypes.data, ptrs, int(time_units),
-                            C.byref(best), log, len(log)), lib)
+                            int(rules), int(sublattice), C.byref(best), log, len(log)), lib)
     return best, log.value.decode()
 
 
@@ -118,7 +125,11 @@ def load_elp():
         lib.elp_run_learned.argtypes = [P, P, I, P, U64, U32, U32, C.POINTER(U64), C.POINTER(U64)]
         lib.elp_net_patch_moves.argtypes = [I, P, P, I, P, P, P]
         lib.elp_net_patch_features.argtypes = [I, P, I, P]
-        lib.elp_autotune.argtypes = [I, I, I, P, P, I, C.POINTER(Schedule), C.c_char_p, U64]
+        lib.elp_autotune.argtypes = [I, I, I, P, P, I, I, I, C.POINTER(Schedule), C.c_char_p, U64]
+        lib.elp_count_moves.argtypes = [P, I]
+        lib.elp_move_counts.argtypes = [P, P]
+        lib.elp_batch_count_moves.argtypes = [P, I]
+        lib.elp_batch_move_counts.argtypes = [P, P]
         lib.elp_schedule_format.argtypes = [C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_schedule_parse.argtypes = [C.c_char_p, C.POINTER(Schedule)]
         lib._elp_typed = True
@@ -216,6 +227,16 @@ class Engine:
     def set_rules(self, rules):
         """ring move rates: RULES_REFERENCE (default) or RULES_BALANCED"""
         _check(self.lib.elp_set_rules(self.h, int(rules)), self.lib)
+
+    def count_moves(self, enable=True):
+        """Launch the counting build of the kernel in the following runs (same moves); enable clears the counts."""
+        _check(self.lib.elp_count_moves(self.h, int(bool(enable))), self.lib)
+
+    def move_counts(self):
+        """{"attempts", "empty", "face", "accepted"} since count_moves(True)."""
+        c = np.zeros(4, np.uint64)
+        _check(self.lib.elp_move_counts(self.h, c.ctypes.data), self.lib)
+        return dict(zip(("attempts", "empty", "face", "accepted"), (int(x) for x in c)))
 
     def ring_chains(self, host):
         """Rings of the device lattice, recovered on the device: (bonds[n_pol][N], lengths, starts[n_pol][3])
@@ -346,6 +367,15 @@ class Batch:
 
     def set_rules(self, rules):
         _check(self.lib.elp_batch_set_rules(self.h, int(rules)), self.lib)
+
+    def count_moves(self, enable=True):
+        _check(self.lib.elp_batch_count_moves(self.h, int(bool(enable))), self.lib)
+
+    def move_counts(self):
+        """{"attempts", "empty", "face", "accepted"} over all replicas since count_moves(True)."""
+        c = np.zeros(4, np.uint64)
+        _check(self.lib.elp_batch_move_counts(self.h, c.ctypes.data), self.lib)
+        return dict(zip(("attempts", "empty", "face", "accepted"), (int(x) for x in c)))
 
 
 NET_NPARAM = 14416

@@ -82,6 +82,21 @@ static int unit_ok(uint32_t u) { return !(u == 0x0 || u == 0x3 || u == 0xc || u 
    reads this byte) */
 typedef struct { uint32_t fixed; } LDraw;
 
+/* Outcome counts of a replay (orc_lin_counts): attempts at an empty site and
+   accepted moves (the move wrote), as the engine's counting kernels report
+   them. */
+static int g_counting = 0;
+static uint64_t g_counts[2];
+
+/* 1: start counting (counts cleared); 0: stop.  out (may be NULL): empty,
+   accepted since the last start. */
+int orc_lin_counts(int enable, uint64_t out[2]) {
+    if (out) { out[0] = g_counts[0]; out[1] = g_counts[1]; }
+    if (enable) g_counts[0] = g_counts[1] = 0;
+    g_counting = enable;
+    return 0;
+}
+
 /* Test-only planted mistakes (orc_lin_plant), to show that the rate checks
    can tell a wrong gate; 0 = the rules as written. */
 static int g_plant = 0;
@@ -91,7 +106,7 @@ int orc_lin_plant(int which) {
     return 0;
 }
 
-static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
+static int trans_move(LTile* X, uint32_t site, LDraw* rd) {
     uint32_t* lattice = X->lattice;
     uint32_t* slab = X->slab;
     uint32_t next, prev, bond, temp, oldNextSite, oldPrevSite, bondSite, newSite, nNext, nPrev, prevBond;
@@ -102,7 +117,7 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
     prev = nextPrev >> 4;
     next = nextPrev & 0xf;
     slabSite = slab[site / 8];
-    if (!nextPrev) return;
+    if (!nextPrev) return 0;
     uint32_t rnd = rd->fixed;
     if ((slabSite >> (16 | (site & 0x7))) & 0x1) {
         prevBond = (rnd & 0x4) >> 2;
@@ -111,11 +126,11 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
             prevBond ^= 0x1;
             newBond = (rnd >> 3) & 0xf;
             insBond = newBond ^ (0xf * (prevBond ^ 0x1));
-            if (!unit_ok(newBond)) return;
-            if (edge_outside(X, site, newBond)) return;
+            if (!unit_ok(newBond)) return 0;
+            if (edge_outside(X, site, newBond)) return 0;
             newSite = add_unit(X, newBond, site);
             latNew = lattice[newSite / 4];
-            if (latNew & (0xffu << ((newSite & 0x3) * 8))) return;
+            if (latNew & (0xffu << ((newSite & 0x3) * 8))) return 0;
             label = (slabSite >> ((site & 0x7) * 2)) & 0x3;
             slabSite &= ~(0x3u << ((site & 0x7) * 2));
             slabSite |= ((label >> prevBond) & 0x1) << ((site & 0x7) * 2);
@@ -129,9 +144,9 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
             if (site / 4 == newSite / 4) latNew = lattice[newSite / 4];
             latNew |= insBond << ((newSite & 0x3) * 8 + 4 * prevBond);
             lattice[newSite / 4] = latNew;
-            return;
+            return 1;
         }
-        if (edge_outside(X, site, bond ^ (0xf * prevBond))) return;
+        if (edge_outside(X, site, bond ^ (0xf * prevBond))) return 0;
         nPrev = (X->tr[bond / 4] >> (4 * (2 * (bond % 4) + (rnd & 0x1)))) & 0xf;
         nNext = uadd((~nPrev) & 0xf, bond);
         temp = nPrev;
@@ -142,10 +157,10 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
         oldNextSite = prevBond ? site : bondSite;
         oldPrevSite = prevBond ? bondSite : site;
         (void)oldNextSite;
-        if (edge_outside(X, oldPrevSite, nPrev)) return;
+        if (edge_outside(X, oldPrevSite, nPrev)) return 0;
         newSite = add_unit(X, nPrev, oldPrevSite);
         latNew = lattice[newSite / 4];
-        if (latNew & (0xffu << ((newSite & 0x3) * 8))) return;
+        if (latNew & (0xffu << ((newSite & 0x3) * 8))) return 0;
         latNew |= (nNext | (nPrev << 4)) << ((newSite & 0x3) * 8);
         latSiteComplete &= ~(0xfu << ((site & 0x3) * 8 + 4 * prevBond));
         latSiteComplete |= (prevBond ? (nNext << 4) : nPrev) << ((site & 0x3) * 8);
@@ -169,15 +184,15 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
         slab[newSite / 8] |= label << ((newSite & 0x7) * 2);
     } else {
         int end = (next == 0 || prev == 0);
-        if (((rnd >> 4) & 0x1) != 0 && !(g_plant == 2 && !end)) return;
+        if (((rnd >> 4) & 0x1) != 0 && !(g_plant == 2 && !end)) return 0;
         if (end) {
-            if (((rnd >> 5) & (g_plant == 1 ? 0x3 : 0x7)) != 0) return;
+            if (((rnd >> 5) & (g_plant == 1 ? 0x3 : 0x7)) != 0) return 0;
             if (next == 0) { bond = prev ^ 0xf; prevBond = 1; }
             else { bond = next; prevBond = 0; }
-            if (edge_outside(X, site, bond)) return;
+            if (edge_outside(X, site, bond)) return 0;
             newSite = add_unit(X, bond, site);
             slabBond = slab[newSite / 8];
-            if (slabBond & (1u << (16 | (newSite & 0x7)))) return;
+            if (slabBond & (1u << (16 | (newSite & 0x7)))) return 0;
             label = (slabSite >> ((site & 0x7) * 2)) & 0x1;
             slabSite &= ~(1u << ((site & 0x7) * 2));
             slab[site / 8] = slabSite;
@@ -193,17 +208,17 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
             latNew = lattice[newSite / 4];
             latNew &= ~(0xfu << ((newSite & 0x3) * 8 + 4 * (prevBond ^ 0x1)));
             lattice[newSite / 4] = latNew;
-            return;
+            return 1;
         }
         prevBond = rnd & 0x1;
-        if (edge_outside(X, site, (~prev) & 0xf) | edge_outside(X, site, next)) return;
-        if (!uvalid_add(next, prev, &bond)) return;
+        if (edge_outside(X, site, (~prev) & 0xf) | edge_outside(X, site, next)) return 0;
+        if (!uvalid_add(next, prev, &bond)) return 0;
         oldNextSite = add_unit(X, next, site);
         oldPrevSite = add_unit(X, (~prev) & 0xf, site);
         bondSite = prevBond ? oldPrevSite : oldNextSite;
-        if (bondSite == site) return;
+        if (bondSite == site) return 0;
         slabBond = slab[bondSite / 8];
-        if (slabBond & (1u << ((bondSite & 0x7) | 16))) return;
+        if (slabBond & (1u << ((bondSite & 0x7) | 16))) return 0;
         slab[bondSite / 8] = slabBond | (1u << ((bondSite & 0x7) | 16));
         slabSite = slab[site / 8];
         label = (slabSite >> (2 * (site & 0x7))) & 0x1;
@@ -224,15 +239,16 @@ static void trans_move(LTile* X, uint32_t site, LDraw* rd) {
         latNext |= (bond << ((oldNextSite % 4) * 8 + 4));
         lattice[oldNextSite / 4] = latNext;
     }
+    return 1;
 }
 
-static void diffuse_sl(LTile* X, uint32_t site) {
+static int diffuse_sl(LTile* X, uint32_t site) {
     uint32_t* lattice = X->lattice;
     uint32_t* slab = X->slab;
     uint32_t next, nextSite, tsl, bit, nBit, lBit, lnBit, slabSite, slabNext, label;
     next = (lattice[site >> 2] >> (8 * (site & 0x3))) & 0xff;
     next &= 0xf;
-    if (!next || edge_outside(X, site, next)) return;
+    if (!next || edge_outside(X, site, next)) return 0;
     nextSite = add_unit(X, next, site);
     slabSite = slab[site / 8];
     slabNext = slab[nextSite / 8];
@@ -253,7 +269,7 @@ static void diffuse_sl(LTile* X, uint32_t site) {
         slabSite &= ~(1u << lBit);
         slabSite |= label << lBit;
     } else {
-        return;
+        return 0;
     }
     tsl = ((slabSite >> bit) & 0x1) << nBit;
     slabSite &= ~(1u << bit);
@@ -266,6 +282,7 @@ static void diffuse_sl(LTile* X, uint32_t site) {
         slab[site / 8] = slabSite;
         slab[nextSite / 8] = slabNext;
     }
+    return 1;
 }
 
 static void gpu_trans(uint32_t tr[4]) {
@@ -342,11 +359,14 @@ int orc_lin_run(uint16_t* lat, int LT, int LU, int LV, int TT, int TU, int TV, i
                                 if (sublat) { t += j; u += (int)((c + 3u * j) % 4); v += (int)(((c >> 2) + 2u * j) % 4); }
                                 else { t += (int)(c & 3); u += (int)((c >> 2) & 3); v += (int)((c >> 4) & 3); }
                                 uint32_t site = lidx(&X, t, u, v);
-                                if (s == 0) diffuse_sl(&X, site);
+                                const int empty = ((X.lattice[site / 4] >> (8 * (site & 3))) & 0xff) == 0;
+                                int acc;
+                                if (s == 0) acc = diffuse_sl(&X, site);
                                 else {
                                     LDraw rd = {(philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> 8) & 0xff};
-                                    trans_move(&X, site, &rd);
+                                    acc = trans_move(&X, site, &rd);
                                 }
+                                if (g_counting) { g_counts[0] += (uint64_t)empty; g_counts[1] += (uint64_t)acc; }
                             }
                         }
                     }

@@ -93,22 +93,22 @@ static int step(const Tile* X, int t, int u, int v, uint32_t bond, int* nt, int*
     return *nt + X->T * (*nu + X->U * *nv);
 }
 
-static void trans_forw(Tile* X, int t, int u, int v, Draw* rd) {
+static int trans_forw(Tile* X, int t, int u, int v, Draw* rd) {
     int x = t + X->T * (u + X->U * v), at, au, av;
     uint32_t s = X->b[x];
-    if (!s) return;
+    if (!s) return 0;
     uint32_t nxt = s & 0xf, lab = s & 0x30, sl = s & 0x40;
     int n = step(X, t, u, v, nxt, &at, &au, &av);
-    if (n < 0) return;
+    if (n < 0) return 0;
     uint32_t ns = X->b[n], nsl = ns & 0x40;
-    if (sl + nsl == 0) return;
+    if (sl + nsl == 0) return 0;
     uint32_t r = draw(rd);
     uint32_t b1 = (X->tr[nxt / 4] >> (4 * (2 * (nxt % 4) + (r & 1)))) & 0xf;
     uint32_t b2 = badd((~b1) & 0xf, nxt);
     uint32_t first = (r & 2) ? b1 : b2, second = (r & 2) ? b2 : b1;
     int d = step(X, t, u, v, first, &at, &au, &av);
-    if (d < 0) return;
-    if (X->b[d]) return;
+    if (d < 0) return 0;
+    if (X->b[d]) return 0;
     int moveFirst = (sl + nsl == 0x80) ? (int)((r & 4) >> 2) : (sl ? 1 : 0);
     uint32_t dn = second;
     if (moveFirst) {
@@ -122,29 +122,30 @@ static void trans_forw(Tile* X, int t, int u, int v, Draw* rd) {
     X->b[x] = (uint8_t)s;
     X->b[d] = (uint8_t)dn;
     if (!moveFirst) X->b[n] = (uint8_t)ns;
+    return 1;
 }
 
-static void trans_back(Tile* X, int t, int u, int v, Draw* rd) {
+static int trans_back(Tile* X, int t, int u, int v, Draw* rd) {
     int x = t + X->T * (u + X->U * v), at, au, av;
     uint32_t s = X->b[x];
     uint32_t nxt = s & 0xf, lab = s & 0x30, sl = s & 0x40;
-    if (!s) return;
+    if (!s) return 0;
     int src = step(X, t, u, v, nxt, &at, &au, &av);
-    if (src < 0) return;
+    if (src < 0) return 0;
     uint32_t ss = X->b[src];
     uint32_t srcNext = ss & 0xf, srcLab = ss & 0x30;
-    if (ss & 0x40) return;
-    if ((nxt | srcNext) != 0xf && (nxt & srcNext)) return;
+    if (ss & 0x40) return 0;
+    if ((nxt | srcNext) != 0xf && (nxt & srcNext)) return 0;
     uint32_t nn = badd(nxt, srcNext);
-    if (nn == 0x3 || nn == 0xc) return;
+    if (nn == 0x3 || nn == 0xc) return 0;
     int w = step(X, t, u, v, nn, &at, &au, &av);
-    if (w < 0) return;
+    if (w < 0) return 0;
     uint32_t ws = X->b[w], wsl = ws & 0x40;
-    if (sl + wsl == 0x80) return;
+    if (sl + wsl == 0x80) return 0;
     uint32_t r = draw(rd);
     /* balanced rates: a fold forced by stored length at one end is accepted
        with probability 1/4, the rate of the TransForw that undoes it */
-    if (X->rules && (sl | wsl) && (r & 6)) return;
+    if (X->rules && (sl | wsl) && (r & 6)) return 0;
     int moveFirst = (sl + wsl == 0) ? (int)(r & 1) : (sl == 0x40 ? 0 : 1);
     if (moveFirst) {
         s = nn | (lab << 1) | srcLab | 0x40;
@@ -155,17 +156,18 @@ static void trans_back(Tile* X, int t, int u, int v, Draw* rd) {
     X->b[src] = 0;
     X->b[x] = (uint8_t)s;
     X->b[w] = (uint8_t)ws;
+    return 1;
 }
 
-static void diffuse_sl(Tile* X, int t, int u, int v) {
+static int diffuse_sl(Tile* X, int t, int u, int v) {
     int x = t + X->T * (u + X->U * v), at, au, av;
     uint32_t s = X->b[x];
     uint32_t nxt = s & 0xf, lab = s & 0x30, sl = s & 0x40;
-    if (!s) return;
+    if (!s) return 0;
     int n = step(X, t, u, v, nxt, &at, &au, &av);
-    if (n < 0) return;
+    if (n < 0) return 0;
     uint32_t ns = X->b[n], nlab = ns & 0x30, nsl = ns & 0x40;
-    if (nsl + sl != 0x40) return;
+    if (nsl + sl != 0x40) return 0;
     if (sl) {
         ns = ns | ((lab & 0x10) << 1) | 0x40;
         s = nxt | ((lab >> 1) & 0x10);
@@ -175,6 +177,42 @@ static void diffuse_sl(Tile* X, int t, int u, int v) {
     }
     X->b[x] = (uint8_t)s;
     X->b[n] = (uint8_t)ns;
+    return 1;
+}
+
+/* ---------------- move counting ----------------
+ * Outcome counts of a replay (orc_ring_counts), the quantities the engine's
+ * counting kernels report: attempts at an empty site; attempts whose move
+ * would touch a site outside the tile -- TransForw: the bond partner or the
+ * target common neighbour (first bond of the pair the random bits pick);
+ * DiffuseSL: the partner; TransBack: the source or the fold target beyond it
+ * -- whatever the move's other conditions; accepted moves (the move wrote). */
+static int g_counting = 0;
+static uint64_t g_counts[3];
+
+static int leaves_tile(const Tile* X, int t, int u, int v, int type, uint32_t r) {
+    uint32_t s = X->b[t + X->T * (u + X->U * v)];
+    uint32_t nxt = s & 0xf;
+    int at, au, av, bt, bu, bv;
+    int n = step(X, t, u, v, nxt, &at, &au, &av);
+    if (n < 0) return 1;
+    if (type == 1) return 0;
+    if (type == 0) {
+        uint32_t b1 = (X->tr[nxt / 4] >> (4 * (2 * (nxt % 4) + (r & 1)))) & 0xf;
+        uint32_t first = (r & 2) ? b1 : badd((~b1) & 0xf, nxt);
+        return step(X, t, u, v, first, &bt, &bu, &bv) < 0;
+    }
+    uint32_t srcNext = X->b[n] & 0xf;
+    return step(X, at, au, av, srcNext, &bt, &bu, &bv) < 0;
+}
+
+/* 1: start counting (counts cleared); 0: stop.  out (may be NULL): empty,
+   leaving the tile, accepted since the last start. */
+int orc_ring_counts(int enable, uint64_t out[3]) {
+    if (out) { out[0] = g_counts[0]; out[1] = g_counts[1]; out[2] = g_counts[2]; }
+    if (enable) g_counts[0] = g_counts[1] = g_counts[2] = 0;
+    g_counting = enable;
+    return 0;
 }
 
 /*
@@ -264,9 +302,17 @@ int orc_ring_run_rules(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int
                                    balanced rates bits 6-7 of the byte: 0,1 TF, 2 DSL, 3 TB */
                                 int mv = s == 1 ? 1 : (s == 3 ? 2 : 0);
                                 if (rules) mv = (rd.fixed >> 6) < 2 ? 0 : (int)(rd.fixed >> 6) - 1;
-                                if (mv == 1) diffuse_sl(&X, t, u, v);
-                                else if (mv == 2) trans_back(&X, t, u, v, &rd);
-                                else trans_forw(&X, t, u, v, &rd);
+                                const int empty = X.b[t + TT * (u + TU * v)] == 0;
+                                const int face = !empty && leaves_tile(&X, t, u, v, mv == 0 ? 0 : (mv == 1 ? 1 : 2), rd.fixed);
+                                int acc;
+                                if (mv == 1) acc = diffuse_sl(&X, t, u, v);
+                                else if (mv == 2) acc = trans_back(&X, t, u, v, &rd);
+                                else acc = trans_forw(&X, t, u, v, &rd);
+                                if (g_counting) {
+                                    g_counts[0] += (uint64_t)empty;
+                                    g_counts[1] += (uint64_t)face;
+                                    g_counts[2] += (uint64_t)(!empty && !face && acc);
+                                }
                             }
                         }
                     }

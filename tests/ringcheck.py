@@ -85,15 +85,23 @@ def oracle():
     return _orc
 
 
-def replay_ring(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0, rules=0):
+def replay_ring(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0, rules=0, counts=False):
+    """The oracle's replay; with counts=True also (empty, leaving the tile, accepted) of its attempts."""
     lat = np.ascontiguousarray(lat, dtype=np.uint8).copy()
     o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
     lib = oracle()
     lib.orc_ring_run_rules.argtypes = [C.c_void_p] + [C.c_int] * 9 + [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                                                        C.c_int, C.c_int]
+    lib.orc_ring_counts.argtypes = [C.c_int, C.c_void_p]
+    if counts:
+        lib.orc_ring_counts(1, None)
     rc = lib.orc_ring_run_rules(lat.ctypes.data, *L, *tile, int(sublat), int(n_iter), len(o), o.ctypes.data,
                                 int(tau0), int(seed), int(stream), int(order), int(rules))
     assert rc == 0
+    if counts:
+        c = np.zeros(3, np.uint64)
+        lib.orc_ring_counts(0, c.ctypes.data)
+        return lat, tuple(int(x) for x in c)
     return lat
 
 
@@ -149,15 +157,23 @@ def lin_label_sequences(lat, L):
 _lin_typed = False
 
 
-def replay_lin(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0):
+def replay_lin(lat, L, tile, sublat, n_iter, origins, tau0, seed, stream=0, order=0, counts=False):
+    """The oracle's replay; with counts=True also (empty, accepted) of its attempts."""
     global _lin_typed
     lib = oracle()
     if not _lin_typed:
         lib.orc_lin_run.argtypes = [C.c_void_p] + [C.c_int] * 9 + [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]
+        lib.orc_lin_counts.argtypes = [C.c_int, C.c_void_p]
         _lin_typed = True
     lat = np.ascontiguousarray(lat, dtype=np.uint16).copy()
     o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+    if counts:
+        lib.orc_lin_counts(1, None)
     rc = lib.orc_lin_run(lat.ctypes.data, *L, *tile, int(sublat), int(n_iter), len(o), o.ctypes.data, int(tau0),
                          int(seed), int(stream), int(order))
     assert rc == 0
+    if counts:
+        c = np.zeros(2, np.uint64)
+        lib.orc_lin_counts(0, c.ctypes.data)
+        return lat, tuple(int(x) for x in c)
     return lat

@@ -78,15 +78,21 @@ def test_batch_error_paths_without_gpu_calls():
 
 
 def test_schedule_record_round_trip():
-    s = elp.Schedule(16, 16, 16, elp.SUBLAT_DENSE, 13, 256, 16, 5.25e11)
+    s = elp.Schedule(16, 16, 16, elp.SUBLAT_DENSE, 13, 256, 16, 5.25e11, elp.RULES_BALANCED, 1.5e11)
     rec = s.record()
-    assert rec == "tile=16x16x16 sublattice=dense n_iter=13 threads=256 units_per_graph=16 attempts_per_s=5.250000e+11"
+    assert rec == ("tile=16x16x16 sublattice=dense n_iter=13 threads=256 units_per_graph=16 attempts_per_s=5.250000e+11"
+                   " rules=balanced accepted_per_s=1.500000e+11")
     t = elp.Schedule.parse(rec)
     assert (t.tile, t.sublattice, t.n_iter, t.threads, t.units_per_graph) == ((16, 16, 16), 1, 13, 256, 16)
-    assert t.attempts_per_s == 5.25e11
+    assert (t.attempts_per_s, t.rules, t.accepted_per_s) == (5.25e11, elp.RULES_BALANCED, 1.5e11)
+    # records written before the rules and accepted-move fields: reference rates
     u = elp.Schedule.parse("tile=32x32x32 sublattice=cubic n_iter=51 threads=256 units_per_graph=0")
-    assert (u.tile, u.sublattice, u.n_iter) == ((32, 32, 32), 0, 51)
-    for bad in ("tile=16x16 sublattice=dense", "tile=16x16x16 sublattice=sparse n_iter=1 threads=64 units_per_graph=0"):
+    assert (u.tile, u.sublattice, u.n_iter, u.rules, u.accepted_per_s) == ((32, 32, 32), 0, 51, 0, 0.0)
+    v = elp.Schedule.parse("tile=16x16x16 sublattice=dense n_iter=13 threads=256 units_per_graph=16 "
+                           "attempts_per_s=5.25e11")
+    assert (v.attempts_per_s, v.rules) == (5.25e11, elp.RULES_REFERENCE)
+    for bad in ("tile=16x16 sublattice=dense", "tile=16x16x16 sublattice=sparse n_iter=1 threads=64 units_per_graph=0",
+                "tile=16x16x16 sublattice=dense n_iter=13 threads=256 units_per_graph=16 attempts_per_s=1 rules=fast"):
         try:
             elp.Schedule.parse(bad)
         except elp.ElpError:

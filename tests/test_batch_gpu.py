@@ -110,22 +110,31 @@ def test_ring_batch_64_replicas_full_size():
     bat.close()
 
 
-@pytest.mark.parametrize("kind", ["ring", "linear"])
-def test_autotuned_schedule_replays_bit_exact(kind):
-    """The tuner's record, parsed back, drives a batch whose replicas equal the
-    C replay of that recorded schedule."""
-    _say(f"[autotune] {kind}")
+@pytest.mark.parametrize("kind,rules", [("ring", elp.RULES_REFERENCE), ("linear", elp.RULES_REFERENCE),
+                                        ("ring", elp.RULES_BALANCED)])
+def test_autotuned_schedule_replays_bit_exact(kind, rules):
+    """The tuner ranks by accepted moves per second among schedules with the
+    statics of the request (reference rates: the dense sub-lattice only;
+    balanced rates: both); its record, parsed back, drives a batch whose
+    replicas equal the C replay of that recorded schedule."""
+    _say(f"[autotune] {kind} rules={rules}")
     host_cls, replay = (elp.RingHost, replay_ring) if kind == "ring" else (elp.LinHost, replay_lin)
     spec = [(100, 32, 11), (200, 64, 12)]
     hosts = [host_cls(N=N, L=L, density=1.2, seed=s) for N, L, s in spec]
     lats = [h.lattice() for h in hosts]
-    best, log = elp.autotune(kind, lats, [h.L for h in hosts], time_units=8)
+    best, log = elp.autotune(kind, lats, [h.L for h in hosts], time_units=8, rules=rules)
     _say(log)
     assert all(np.array_equal(h.lattice(), l) for h, l in zip(hosts, lats))     # inputs untouched
-    assert best.attempts_per_s > 0 and "chosen:" in log
+    assert best.attempts_per_s > 0 and 0 < best.accepted_per_s < best.attempts_per_s and "chosen:" in log
+    kernels = [ln for ln in log.splitlines() if ln.startswith("kernel ")]
+    subs = {ln.split("sublattice=")[1].split()[0] for ln in kernels}
+    assert subs == ({"dense"} if rules == elp.RULES_REFERENCE else {"dense", "cubic"}), subs
+    assert all("accepted/s" in ln for ln in kernels)
     s = elp.Schedule.parse(best.record())
+    assert s.rules == rules and s.accepted_per_s > 0
     bat = elp.Batch(kind, [h.L for h in hosts], s.tile, s.sublattice, s.n_iter)
     bat.set_threads(s.threads)
+    bat.set_rules(rules)
     for r, h in enumerate(hosts):
         bat.upload(r, lats[r])
         h.load_shift(s.tile)
@@ -133,7 +142,8 @@ def test_autotuned_schedule_replays_bit_exact(kind):
     org = np.stack([h.origins(s.tile, n_time) for h in hosts], axis=1)
     bat.run(org, tau0=5, seeds=[h.seed for h in hosts], streams=[0, 0], units_per_graph=s.units_per_graph)
     for r, h in enumerate(hosts):
-        want = replay(lats[r], h.L, s.tile, s.sublattice, s.n_iter, org[:, r], 5, h.seed, 0)
+        kw = {"rules": rules} if kind == "ring" else {}
+        want = replay(lats[r], h.L, s.tile, s.sublattice, s.n_iter, org[:, r], 5, h.seed, 0, **kw)
         assert np.array_equal(bat.download(r), want), f"replica {r}"
     bat.close()
 
