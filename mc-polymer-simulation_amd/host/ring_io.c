@@ -42,6 +42,7 @@ int ReadLatticeFileUnit(SimState* ss, SimProperties* sp, const char* file, int u
     char* text = (char*)malloc((size_t)maxLen + 2);
     char fmt[32];
     snprintf(fmt, sizeof fmt, "%%%ds", maxLen + 1);
+    int longest = 0;
     for (int p = 0; p < np; p++) {
         int len; uint t, u, v;
         if (fscanf(f, "%*s %i", &len) != 1 || fscanf(f, "%u %u %u", &t, &u, &v) != 3 || fscanf(f, fmt, text) != 1 ||
@@ -54,6 +55,7 @@ int ReadLatticeFileUnit(SimState* ss, SimProperties* sp, const char* file, int u
             bonds[i] = (c >= '0' && c <= '9') ? (uint)(c - '0') : (uint)(10 + (c | 0x20) - 'a');
             if (bonds[i] > 0xf) { printf("Error: bad bond character in %s\n", file); fclose(f); free(bonds); free(text); return 199; }
         }
+        if (len > longest) longest = len;
         if (AddPolymerToSim(ss, sp, t, u, v, len, bonds)) {
             printf("Error: polymer %i of %s overlaps another one\n", p, file);
             fclose(f); free(bonds); free(text); return 199;
@@ -61,6 +63,9 @@ int ReadLatticeFileUnit(SimState* ss, SimProperties* sp, const char* file, int u
     }
     fclose(f);
     free(bonds); free(text);
+    /* files whose header gives the chain length itself as maxPolLength (efpol
+       writes them so) would stop the chain walk one monomer short */
+    if (longest >= (int)sp->maxPolLength) sp->maxPolLength = (uint)longest + 1;
     printf("Succesfully read file %s\n", file);
     printf("Finished Reading lattice file\n");
     return 0;

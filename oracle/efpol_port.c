@@ -27,6 +27,13 @@
  *   efpol_port SEED DIR DENSITY TIME INTERVAL NMONO DBL_STEP L MODEL
  *       same argv as the reference (cpupol/efpol.c:22-38); writes the same
  *       files DIR/N_<NMONO>/t=<T>_dev=0.res and simulation_settings.txt.
+ *   efpol_port from TUV SEED DIR TIME INTERVAL MAXSTACK
+ *       efpol's sl_equal dynamics from the chains of a tuv file; MAXSTACK 2
+ *       rejects every move that would put a third monomer on a site, so the
+ *       chain samples the state space of the GPU engines (at most one
+ *       stored-length monomer per site); 0 = efpol itself.
+ *   efpol_port table MODEL
+ *       prints the transition table, one slot per line.
  *   efpol_port bench L NMONO DENSITY SEED NTIME [NWARM]
  *       no files; runs NWARM untimed time units, then times NTIME units of
  *       the DoMCStep region only (cpupol/efpol.c:92-95) and prints
@@ -92,6 +99,7 @@ typedef struct {
     int ntrans[16 * 16];
     int add_occ[16 * 16 * NSLOT];
     int rem_occ[16 * 16 * NSLOT];
+    int maxStack;          /* 2: reject moves that put a third monomer on a site (0: no limit, efpol itself) */
 } Sim;
 
 static uint32_t site_of(const Sim* S, int t, int u, int v) { return (uint32_t)(t + S->L * (u + S->L * v)); }
@@ -256,6 +264,16 @@ static long mc_step(Sim* S) {
         uint32_t e = S->trans[opt];
         uint32_t d1 = e & 0xf, d2 = (e >> 4) & 0xf;
         if (!d1 && !d2) continue;
+        if (S->maxStack == 2) {
+            /* a site holds three monomers iff two consecutive bonds are both 0 */
+#ifdef POL_LIN
+            int pp = prev - 1, nn = mono + 1;
+            uint32_t bpp = pp >= 0 ? unit_get(S, pp, p) : 0xf, bnn = nn < N ? unit_get(S, nn, p) : 0xf;
+#else
+            uint32_t bpp = unit_get(S, (prev - 1 + N) % N, p), bnn = unit_get(S, (mono + 1) % N, p);
+#endif
+            if ((d1 == 0 && (bpp == 0 || d2 == 0)) || (d2 == 0 && bnn == 0)) continue;
+        }
         uint32_t* ps = &S->pos[p + mono * S->nPol];
         uint32_t to = site_step(S, *ps, e >> 8);
         if (S->add_occ[opt]) {
@@ -289,6 +307,44 @@ static void write_tuv(const Sim* S, const char* file) {
         fprintf(f, "\n");
     }
     fclose(f);
+}
+
+static int bl_for(int L);
+
+/* Start from a tuv file (box edge a power of two, every chain of the same
+   length): bonds as given, monomer positions walked from each start.
+   Returns 0, or -1 on a file this reader does not take. */
+static int read_tuv(Sim* S, const char* file) {
+    FILE* f = fopen(file, "r");
+    if (!f) return -1;
+    int LT, LU, LV, np, maxLen;
+    if (fscanf(f, "%*s %d %*s %d %*s %d %*s %d %*s %d", &LT, &LU, &LV, &np, &maxLen) != 5 || LT != LU || LT != LV ||
+        np <= 0 || maxLen <= 0 || (LT & (LT - 1))) { fclose(f); return -1; }
+    char* text = malloc((size_t)maxLen + 2);
+    char fmt[32];
+    snprintf(fmt, sizeof fmt, "%%%ds", maxLen + 1);
+    int rc = 0;
+    for (int p = 0; p < np && !rc; p++) {
+        int len; unsigned t, u, v;
+        if (fscanf(f, "%*s %d %u %u %u", &len, &t, &u, &v) != 4 || fscanf(f, fmt, text) != 1 || (int)strlen(text) != len) {
+            rc = -1; break;
+        }
+        if (p == 0) sim_alloc(S, bl_for(LT), len, np);
+        if (len != S->polSize) { rc = -1; break; }
+        uint32_t site = site_of(S, (int)t, (int)u, (int)v);
+        for (int m = 0; m < len; m++) {
+            char c = text[m];
+            uint32_t b = (c >= '0' && c <= '9') ? (uint32_t)(c - '0') : (uint32_t)(10 + (c | 0x20) - 'a');
+            if (b > 0xf) { rc = -1; break; }
+            S->pos[p + S->nPol * m] = site;
+            occ_set(S, site);
+            unit_or(S, m, p, b);
+            if (b && b != 0xf) site = site_step(S, site, b);
+        }
+    }
+    free(text);
+    fclose(f);
+    return rc;
 }
 
 static void write_settings(const Sim* S, const char* dir, uint32_t seed, double density,
@@ -341,6 +397,42 @@ int main(int argc, char** argv) {
         for (int k = 0; k < 4; k++) s[k] = (uint32_t)strtoul(argv[2 + k], NULL, 10);
         const long n = atol(argv[6]);
         for (long i = 0; i < n; i++) printf("%u\n", lfsr113(s));
+        return 0;
+    }
+    if (argc >= 2 && !strcmp(argv[1], "from")) {
+        /* from TUV SEED DIR TIME INTERVAL MAXSTACK: efpol's sl_equal dynamics from the chains of a tuv
+           file, with at most MAXSTACK (2, or 0 = no limit) monomers per site; writes
+           DIR/N_<len>/t=<T>_dev=0.res every INTERVAL time units like the reference driver */
+        if (argc < 8) { fprintf(stderr, "usage: %s from TUV SEED DIR TIME INTERVAL MAXSTACK\n", argv[0]); return 2; }
+        Sim S; memset(&S, 0, sizeof S);
+        if (read_tuv(&S, argv[2])) { fprintf(stderr, "cannot read %s\n", argv[2]); return 199; }
+        uint32_t seed = (uint32_t)atol(argv[3]);
+        long tMax = atol(argv[5]), interval = atol(argv[6]);
+        S.maxStack = atoi(argv[7]);
+        lfsr113_seed_efpol(S.rng, seed);
+        build_trans(&S, MODEL_EQUAL);
+        char sub[4096], file[8192];
+        snprintf(sub, sizeof sub, "%s/N_%i", argv[4], S.polSize);
+        mkdir_p(sub);
+        for (long t = 0; t <= tMax; t += interval) {
+            snprintf(file, sizeof file, "%s/t=%li_dev=%i.res", sub, t, 0);
+            write_tuv(&S, file);
+            if (t < tMax) mc_run(&S, interval);
+        }
+        return 0;
+    }
+    if (argc >= 2 && !strcmp(argv[1], "table")) {
+        /* table MODEL: every transition slot, "s1 s2 slot d1 d2" in hex (16 slots per bond pair) */
+        if (argc < 3) { fprintf(stderr, "usage: %s table sl_equal|sl_double|sl_quad\n", argv[0]); return 2; }
+        int model = !strcmp(argv[2], "sl_double") ? MODEL_DOUBLE : !strcmp(argv[2], "sl_quad") ? MODEL_QUAD : MODEL_EQUAL;
+        Sim S; memset(&S, 0, sizeof S);
+        S.BL = 5; S.L = 32;
+        build_trans(&S, model);
+        for (int pair = 0; pair < 256; pair++)
+            for (int k = 0; k < NSLOT; k++) {
+                uint32_t e = S.trans[NSLOT * pair + k];
+                printf("%x %x %d %x %x\n", pair >> 4, pair & 15, k, e & 0xf, (e >> 4) & 0xf);
+            }
         return 0;
     }
     if (argc >= 2 && !strcmp(argv[1], "bench")) {

@@ -9,7 +9,16 @@ seeding ignores bit 0 of the seed, so seeds 2k and 2k+1 give the same run),
 over the last 30 frames (elp.stats: Rg^2, stored-length ratio, P(s), g1, g3)
 to ring_L32_N100.json or lin_L32_N100.json next to this script.
 
-    python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear] [workdir]
+Mode ring2 runs efpol's dynamics restricted to at most two monomers per site
+(`efpol_port_ring from START SEED DIR T INTERVAL 2`: every move that would
+stack a third monomer is rejected), from the GPU engine's own start (the
+RingHost melt of seed k, k = 0..7, efpol seed 2k+2), 60000 time units with a
+tuv file every 1000, the first 21 frames discarded, and writes
+ring2_L32_N100.json.  efpol's sl_equal transition table is symmetric, so this
+chain samples the uniform measure over the states a GPU lattice holds: the
+equilibrium of the GPU engine with balanced rates.
+
+    python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear|ring2] [workdir]
 """
 import glob
 import json
@@ -25,6 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "mc-polymer-simulation_amd"))
 import numpy as np  # noqa: E402
 
 from elp.stats import Series  # noqa: E402
+import elp  # noqa: E402
 
 L, N, SEEDS = 32, 100, list(range(2, 18, 2))
 T, INTERVAL = 40000, 1000             # rings; linear chains relax slower: 10x (main)
@@ -45,9 +55,59 @@ def stacking(files):
     return n3 / tot
 
 
+T2, N_THERM2 = 60000, 21
+
+
+def frames_of(d, t_units, interval):
+    files = sorted(glob.glob(os.path.join(d, f"N_{N}", "t=*_dev=0.res")),
+                   key=lambda f: int(os.path.basename(f)[2:].split("_")[0]))
+    assert len(files) == t_units // interval + 1
+    return files
+
+
+def seed_stats(ring, files, n_therm):
+    npol = int(open(files[0]).read().split()[7])
+    ser = Series(ring, (L, L, L), npol, N, len(files))
+    for f in files:
+        ser.add_tuv(f)
+    st = ser.statics(n_therm)
+    g1, g3 = ser.msd(n_therm, max(LAGS) + 1)
+    return {"n_pol": npol, "rg2": st["rg2"], "slrat": st["slrat"], "pc": st["pc"][:N // 2 + 1].tolist(),
+            "g1": g1[LAGS].tolist(), "g3": g3[LAGS].tolist(), "stack3": stacking(files[n_therm:])}
+
+
+def main_ring2(work):
+    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
+    procs, dirs = [], []
+    for k in range(8):
+        d = os.path.join(work, f"k{k}")
+        os.makedirs(d, exist_ok=True)
+        h = elp.RingHost(N=N, L=L, density=1.2, seed=k)
+        start = os.path.join(d, "start.res")
+        h.write_tuv(start)
+        h.close()
+        dirs.append(d)
+        procs.append(subprocess.Popen([exe, "from", start, str(2 * k + 2), d, str(T2), str(INTERVAL), "2"],
+                                      stdout=subprocess.DEVNULL))
+    for p in procs:
+        assert p.wait() == 0
+    out = {"L": L, "N": N, "density": 1.2, "model": "sl_equal, at most two monomers per site", "time_units": T2,
+           "interval": INTERVAL, "n_therm_frames": N_THERM2, "lags": LAGS,
+           "start": "RingHost(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2", "seeds": []}
+    for k, d in enumerate(dirs):
+        row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(True, frames_of(d, T2, INTERVAL), N_THERM2)}
+        assert row["stack3"] == 0
+        out["seeds"].append(row)
+        print(k, row["rg2"], row["slrat"], flush=True)
+    with open(os.path.join(HERE, "ring2_L32_N100.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main():
     args = sys.argv[1:]
-    kind = args.pop(0) if args and args[0] in ("ring", "linear") else "ring"
+    kind = args.pop(0) if args and args[0] in ("ring", "linear", "ring2") else "ring"
+    if kind == "ring2":
+        return main_ring2(args[0] if args else tempfile.mkdtemp())
     ring = kind == "ring"
     work = args[0] if args else tempfile.mkdtemp()
     t_units, interval = (T, INTERVAL) if ring else (10 * T, 10 * INTERVAL)
