@@ -259,6 +259,17 @@ int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int 
    statics and mean-square displacements (elp_series_new_summary). */
 int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
+/* The same frame summary of a linear-chain engine (chains recovered as
+   elp_lin_chains; the middle monomer is (n-1)/2, the analysis' polSize/2 with
+   polSize = n-1 bonds, and a chain's n-1 bonds are its stored-length
+   candidates, analysis/mode_compute.c:145-153). */
+int elp_lin_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                          int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
+/* The same frame summary of replica `rep` of a batch (ring or linear, as the
+   batch), read in place in the batch's lattice buffer on the batch's stream:
+   a series of a batched run needs no lattice download per frame. */
+int elp_batch_frame_summary(elp_batch* b, int rep, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts);
 
 /* elp_ring_frame_summary without allocation.  No reference counterpart: for
    callers that cannot allocate device memory (code under HIP graph capture,

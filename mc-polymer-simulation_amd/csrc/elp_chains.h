@@ -19,13 +19,15 @@ int chains_launch_lin_walk(const uint16_t* lat, int LT, int LU, int LV, const ui
                            hipStream_t st);
 int chains_launch_text(const uint8_t* bonds, const int32_t* lens, const int32_t* start3, const uint64_t* offset,
                        int n_pol, int max_len, char* text, hipStream_t st);
-// Per ring: sum of its monomer positions and its middle monomer relative to
-// the start; contacts[g] += pairs g apart that touch.  n_max = longest ring
-// (<= kSummaryMaxLen: the LDS image is summary_lds_bytes(n_max)).
+// Per chain (ring or linear: a linear chain's last bond 0xf adds no step):
+// sum of its monomer positions and its middle monomer (index n/2 of a ring,
+// (n-1)/2 of a linear chain) relative to the start; contacts[g] += pairs g
+// apart that touch.  n_max = longest chain (<= kSummaryMaxLen: the LDS image
+// is summary_lds_bytes(n_max)).
 constexpr int kSummaryMaxLen = 8192;
 constexpr size_t summary_lds_bytes(int n_max) { return 32 + 4 * 3 * 256 + 16 * (size_t)n_max; }
-int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int64_t* sum3,
-                          int32_t* mid3, unsigned long long* contacts, hipStream_t st);
+int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int linear,
+                          int64_t* sum3, int32_t* mid3, unsigned long long* contacts, hipStream_t st);
 // p[0..n) = v (clears scratch memory without a memset node)
 int chains_launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 // stat[0] = longest chain (atomicMax), stat[1] += chains with no length (lens <= 0)

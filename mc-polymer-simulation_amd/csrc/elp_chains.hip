@@ -240,16 +240,18 @@ constexpr uint32_t contact_mask() {
 constexpr uint32_t kContactMask = contact_mask();
 constexpr int kSumThreads = 256;
 
-// One workgroup per ring (in id order): unwrapped (t,u,v) of every monomer
-// relative to the ring start (a chunked prefix sum of the bond string in
-// LDS), their sum, the middle monomer (index n/2) and the contact histogram:
-// pairs i < j on one site or on nearest-neighbour sites in the ring's own
-// unwrapped frame, counted at j - i (analysis/mode_compute.c:429-504).
+// One workgroup per chain (in id order): unwrapped (t,u,v) of every monomer
+// relative to the chain start (a chunked prefix sum of the bond string in
+// LDS), their sum, the middle monomer (index polSize/2: n/2 for a ring,
+// (n-1)/2 for a linear chain, whose last bond 0xf adds no step) and the
+// contact histogram: pairs i < j on one site or on nearest-neighbour sites in
+// the chain's own unwrapped frame, counted at j - i
+// (analysis/mode_compute.c:429-504).
 // Dynamic LDS: 3 int64 sums, 3 x 256 chunk partials, then x, y, z, hist of
 // n_max words each.
-__global__ __launch_bounds__(kSumThreads) void ring_summary_kernel(const uint8_t* __restrict__ bonds,
+__global__ __launch_bounds__(kSumThreads) void chain_summary_kernel(const uint8_t* __restrict__ bonds,
                                                                    const int32_t* __restrict__ lens, int max_len,
-                                                                   int n_max, int64_t* __restrict__ sum3,
+                                                                   int n_max, int linear, int64_t* __restrict__ sum3,
                                                                    int32_t* __restrict__ mid3,
                                                                    unsigned long long* __restrict__ contacts) {
     extern __shared__ unsigned long long smem[];
@@ -309,21 +311,21 @@ __global__ __launch_bounds__(kSumThreads) void ring_summary_kernel(const uint8_t
         if (hist[g]) atomicAdd(&contacts[g], (unsigned long long)hist[g]);
     if (tid == 0) {
         sum3[3 * id] = (int64_t)ssum[0]; sum3[3 * id + 1] = (int64_t)ssum[1]; sum3[3 * id + 2] = (int64_t)ssum[2];
-        const int m = n / 2;
+        const int m = linear ? (n - 1) / 2 : n / 2;            // monomer polSize / 2
         mid3[3 * id] = px[m]; mid3[3 * id + 1] = py[m]; mid3[3 * id + 2] = pz[m];
     }
 }
 
-int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int64_t* sum3,
-                          int32_t* mid3, unsigned long long* contacts, hipStream_t st) {
+int chains_launch_summary(const uint8_t* bonds, const int32_t* lens, int n_pol, int max_len, int n_max, int linear,
+                          int64_t* sum3, int32_t* mid3, unsigned long long* contacts, hipStream_t st) {
     const size_t lds = summary_lds_bytes(n_max);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)ring_summary_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipError_t e = hipFuncSetAttribute((const void*)chain_summary_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(ring_summary_kernel, dim3(n_pol), dim3(kSumThreads), lds, st, bonds, lens, max_len, n_max, sum3,
-                       mid3, contacts);
+    hipLaunchKernelGGL(chain_summary_kernel, dim3(n_pol), dim3(kSumThreads), lds, st, bonds, lens, max_len, n_max, linear,
+                       sum3, mid3, contacts);
     return (int)hipGetLastError();
 }
 

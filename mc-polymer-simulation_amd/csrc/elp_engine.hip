@@ -935,19 +935,45 @@ int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origin
     return ELP_OK;
 }
 
-// Ring recovery on the device: per ring (in id order) its bond string, length
-// and start site; optionally the tuv text.  Shared by elp_ring_chains and
-// elp_ring_tuv.
-struct ChainBufs {
-    DevBuf l2i, starts, count, bonds, lens, start3, line_len, rg2, nsl, err, offset, text, sum3, mid3, contacts;
-    int n_max = 0;   // longest ring
+// Chain recovery on the device: per chain (in id order) its bond string,
+// length and start site; optionally the tuv text or a frame summary.  Works on
+// a view of one lattice in device memory: an engine's lattice or one replica
+// of a batch.
+struct LatView {
+    int kind;                 // 0 ring, 1 linear
+    int device;
+    const void* lat;
+    int LT, LU, LV;
+    hipStream_t stream;
 };
 
-static int chains_run(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
-    if (!e || !label2id || n_pol <= 0 || max_len <= 0) return fail(ELP_E_ARG, "bad chain-recovery arguments");
+static int engine_view(elp_engine* e, int kind, LatView* v) {
+    if (!e) return fail(ELP_E_ARG, "NULL engine");
     if (e->kind != kind) return fail(ELP_E_ARG, "chain recovery needs a %s engine", kind ? "linear" : "ring");
     if (!e->uploaded) return fail(ELP_E_STATE, "chain recovery before upload");
-    HIPCHK(hipSetDevice(e->device));
+    *v = LatView{e->kind, e->device, e->d_lat, e->LT, e->LU, e->LV, e->stream};
+    return ELP_OK;
+}
+
+static int batch_view(elp_batch* b, int rep, LatView* v) {
+    if (!b) return fail(ELP_E_ARG, "NULL batch");
+    if (rep < 0 || rep >= b->n_rep) return fail(ELP_E_ARG, "replica %d out of range (%d replicas)", rep, b->n_rep);
+    if (!b->uploaded[rep]) return fail(ELP_E_STATE, "chain recovery of replica %d before upload", rep);
+    *v = LatView{b->kind, b->device, (const char*)b->d_lat + b->reps[rep].site_off * b->bytes_per_site,
+                 b->box[3 * rep], b->box[3 * rep + 1], b->box[3 * rep + 2], b->stream};
+    return ELP_OK;
+}
+
+struct ChainBufs {
+    DevBuf l2i, starts, count, bonds, lens, start3, line_len, rg2, nsl, err, offset, text, sum3, mid3, contacts;
+    int n_max = 0;   // longest chain
+};
+
+static int chains_run(const LatView& v, const int32_t* label2id, int n_pol, int max_len, ChainBufs& B) {
+    if (!label2id || n_pol <= 0 || max_len <= 0) return fail(ELP_E_ARG, "bad chain-recovery arguments");
+    const int kind = v.kind;
+    hipStream_t st = v.stream;
+    HIPCHK(hipSetDevice(v.device));
     const size_t nl = (size_t)1 << 18;
     HIPCHK(hipMalloc(&B.l2i.p, sizeof(int32_t) * nl));
     HIPCHK(hipMalloc(&B.starts.p, sizeof(uint32_t) * (size_t)n_pol));
@@ -958,38 +984,38 @@ static int chains_run(elp_engine* e, int kind, const int32_t* label2id, int n_po
     HIPCHK(hipMalloc(&B.line_len.p, sizeof(int32_t) * n_pol));
     HIPCHK(hipMalloc(&B.rg2.p, sizeof(double) * n_pol));
     HIPCHK(hipMalloc(&B.nsl.p, sizeof(int32_t) * n_pol));
-    HIPCHK(hipMemcpyAsync(B.l2i.p, label2id, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(B.l2i.p, label2id, sizeof(int32_t) * nl, hipMemcpyHostToDevice, st));
     uint32_t zero[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(B.count.p, zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(B.count.p, zero, sizeof zero, hipMemcpyHostToDevice, st));
     std::vector<int32_t> neg(n_pol, 0);
-    HIPCHK(hipMemcpyAsync(B.lens.p, neg.data(), sizeof(int32_t) * n_pol, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(B.lens.p, neg.data(), sizeof(int32_t) * n_pol, hipMemcpyHostToDevice, st));
     uint32_t* cnt = (uint32_t*)B.count.p;
-    int rc = kind ? elp::chains_launch_lin_heads((const uint16_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
-                                                 (uint32_t)n_pol, cnt, e->stream)
-                  : elp::chains_launch_heads((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (uint32_t*)B.starts.p,
-                                             (uint32_t)n_pol, cnt, e->stream);
+    int rc = kind ? elp::chains_launch_lin_heads((const uint16_t*)v.lat, v.LT, v.LU, v.LV, (uint32_t*)B.starts.p,
+                                                 (uint32_t)n_pol, cnt, st)
+                  : elp::chains_launch_heads((const uint8_t*)v.lat, v.LT, v.LU, v.LV, (uint32_t*)B.starts.p,
+                                             (uint32_t)n_pol, cnt, st);
     if (rc) return fail(ELP_E_HIP, "head kernel: %s", hipGetErrorString((hipError_t)rc));
     uint32_t nfound = 0;
-    HIPCHK(hipMemcpyAsync(&nfound, cnt, sizeof nfound, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpyAsync(&nfound, cnt, sizeof nfound, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     if (nfound != (uint32_t)n_pol)
         return fail(ELP_E_STATE, "found %u %s, expected %d", nfound, kind ? "chain starts" : "ring heads", n_pol);
     if (kind)
-        rc = elp::chains_launch_lin_walk((const uint16_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p,
+        rc = elp::chains_launch_lin_walk((const uint16_t*)v.lat, v.LT, v.LU, v.LV, (const uint32_t*)B.starts.p,
                                          nfound, (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p,
                                          (int32_t*)B.lens.p, (int32_t*)B.start3.p, (int32_t*)B.line_len.p,
-                                         (double*)B.rg2.p, (int32_t*)B.nsl.p, cnt + 1, e->stream);
+                                         (double*)B.rg2.p, (int32_t*)B.nsl.p, cnt + 1, st);
     else
-        rc = elp::chains_launch_walk((const uint8_t*)e->d_lat, e->LT, e->LU, e->LV, (const uint32_t*)B.starts.p, nfound,
+        rc = elp::chains_launch_walk((const uint8_t*)v.lat, v.LT, v.LU, v.LV, (const uint32_t*)B.starts.p, nfound,
                                      (const int32_t*)B.l2i.p, n_pol, max_len, (uint8_t*)B.bonds.p, (int32_t*)B.lens.p,
                                      (int32_t*)B.start3.p, (int32_t*)B.line_len.p, (double*)B.rg2.p, (int32_t*)B.nsl.p,
-                                     cnt + 1, e->stream);
+                                     cnt + 1, st);
     if (rc) return fail(ELP_E_HIP, "walk kernel: %s", hipGetErrorString((hipError_t)rc));
     uint32_t err = 0;
-    HIPCHK(hipMemcpyAsync(&err, cnt + 1, sizeof err, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(&err, cnt + 1, sizeof err, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> lens(n_pol);
-    HIPCHK(hipMemcpyAsync(lens.data(), B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpyAsync(lens.data(), B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     if (err & 1u) return fail(ELP_E_STATE, "a chain label is not a known polymer id");
     if (err & 2u) return fail(ELP_E_STATE, "a chain is longer than %d", max_len);
     if (err & 4u) return fail(ELP_E_STATE, "a chain is shorter than its 23 head and id labels");
@@ -1002,13 +1028,15 @@ static int chains_run(elp_engine* e, int kind, const int32_t* label2id, int n_po
 
 static int chains_get(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len, uint8_t* bonds,
                       int32_t* lens, int32_t* starts) {
-    ChainBufs B;
-    int rc = chains_run(e, kind, label2id, n_pol, max_len, B);
+    LatView v;
+    int rc = engine_view(e, kind, &v);
     if (rc) return rc;
-    if (bonds) HIPCHK(hipMemcpyAsync(bonds, B.bonds.p, (size_t)n_pol * max_len, hipMemcpyDeviceToHost, e->stream));
-    if (lens) HIPCHK(hipMemcpyAsync(lens, B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    if (starts) HIPCHK(hipMemcpyAsync(starts, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    ChainBufs B;
+    if ((rc = chains_run(v, label2id, n_pol, max_len, B))) return rc;
+    if (bonds) HIPCHK(hipMemcpyAsync(bonds, B.bonds.p, (size_t)n_pol * max_len, hipMemcpyDeviceToHost, v.stream));
+    if (lens) HIPCHK(hipMemcpyAsync(lens, B.lens.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, v.stream));
+    if (starts) HIPCHK(hipMemcpyAsync(starts, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, v.stream));
+    HIPCHK(hipStreamSynchronize(v.stream));
     return ELP_OK;
 }
 
@@ -1024,41 +1052,65 @@ int elp_lin_chains(elp_engine* e, const int32_t* label2id, int n_pol, int max_le
 
 int elp_ring_chain_stats(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, double* rg2,
                          int32_t* n_sl) {
-    ChainBufs B;
-    int rc = chains_run(e, 0, label2id, n_pol, max_len, B);
+    LatView v;
+    int rc = engine_view(e, 0, &v);
     if (rc) return rc;
-    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    ChainBufs B;
+    if ((rc = chains_run(v, label2id, n_pol, max_len, B))) return rc;
+    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, v.stream));
+    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, v.stream));
+    HIPCHK(hipStreamSynchronize(v.stream));
+    return ELP_OK;
+}
+
+static int frame_summary(const LatView& v, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                         int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
+    if (!start3 || !sum3 || !mid3) return fail(ELP_E_ARG, "frame summary needs start3, sum3 and mid3");
+    ChainBufs B;
+    int rc = chains_run(v, label2id, n_pol, max_len, B);
+    if (rc) return rc;
+    if (B.n_max > elp::kSummaryMaxLen)
+        return fail(ELP_E_ARG, "frame summary supports chains of up to %d monomers (found %d)", elp::kSummaryMaxLen,
+                    B.n_max);
+    hipStream_t st = v.stream;
+    HIPCHK(hipMalloc(&B.sum3.p, sizeof(int64_t) * 3 * (size_t)n_pol));
+    HIPCHK(hipMalloc(&B.mid3.p, sizeof(int32_t) * 3 * (size_t)n_pol));
+    HIPCHK(hipMalloc(&B.contacts.p, sizeof(uint64_t) * (size_t)max_len));
+    HIPCHK(hipMemsetAsync(B.contacts.p, 0, sizeof(uint64_t) * (size_t)max_len, st));
+    rc = elp::chains_launch_summary((const uint8_t*)B.bonds.p, (const int32_t*)B.lens.p, n_pol, max_len, B.n_max,
+                                    v.kind, (int64_t*)B.sum3.p, (int32_t*)B.mid3.p, (unsigned long long*)B.contacts.p,
+                                    st);
+    if (rc) return fail(ELP_E_HIP, "summary kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpyAsync(start3, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sum3, B.sum3.p, sizeof(int64_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mid3, B.mid3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
+    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, st));
+    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, st));
+    if (contacts)
+        HIPCHK(hipMemcpyAsync(contacts, B.contacts.p, sizeof(uint64_t) * max_len, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return ELP_OK;
 }
 
 int elp_ring_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
-    if (!start3 || !sum3 || !mid3) return fail(ELP_E_ARG, "frame summary needs start3, sum3 and mid3");
-    ChainBufs B;
-    int rc = chains_run(e, 0, label2id, n_pol, max_len, B);
-    if (rc) return rc;
-    if (B.n_max > elp::kSummaryMaxLen)
-        return fail(ELP_E_ARG, "frame summary supports rings of up to %d monomers (found %d)", elp::kSummaryMaxLen,
-                    B.n_max);
-    HIPCHK(hipMalloc(&B.sum3.p, sizeof(int64_t) * 3 * (size_t)n_pol));
-    HIPCHK(hipMalloc(&B.mid3.p, sizeof(int32_t) * 3 * (size_t)n_pol));
-    HIPCHK(hipMalloc(&B.contacts.p, sizeof(uint64_t) * (size_t)max_len));
-    HIPCHK(hipMemsetAsync(B.contacts.p, 0, sizeof(uint64_t) * (size_t)max_len, e->stream));
-    rc = elp::chains_launch_summary((const uint8_t*)B.bonds.p, (const int32_t*)B.lens.p, n_pol, max_len, B.n_max,
-                                    (int64_t*)B.sum3.p, (int32_t*)B.mid3.p, (unsigned long long*)B.contacts.p,
-                                    e->stream);
-    if (rc) return fail(ELP_E_HIP, "summary kernel: %s", hipGetErrorString((hipError_t)rc));
-    HIPCHK(hipMemcpyAsync(start3, B.start3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(sum3, B.sum3.p, sizeof(int64_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(mid3, B.mid3.p, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, e->stream));
-    if (rg2) HIPCHK(hipMemcpyAsync(rg2, B.rg2.p, sizeof(double) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    if (n_sl) HIPCHK(hipMemcpyAsync(n_sl, B.nsl.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
-    if (contacts)
-        HIPCHK(hipMemcpyAsync(contacts, B.contacts.p, sizeof(uint64_t) * max_len, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return ELP_OK;
+    LatView v;
+    int rc = engine_view(e, 0, &v);
+    return rc ? rc : frame_summary(v, label2id, n_pol, max_len, start3, sum3, mid3, rg2, n_sl, contacts);
+}
+
+int elp_lin_frame_summary(elp_engine* e, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                          int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
+    LatView v;
+    int rc = engine_view(e, 1, &v);
+    return rc ? rc : frame_summary(v, label2id, n_pol, max_len, start3, sum3, mid3, rg2, n_sl, contacts);
+}
+
+int elp_batch_frame_summary(elp_batch* b, int rep, const int32_t* label2id, int n_pol, int max_len, int32_t* start3,
+                            int64_t* sum3, int32_t* mid3, double* rg2, int32_t* n_sl, uint64_t* contacts) {
+    LatView v;
+    int rc = batch_view(b, rep, &v);
+    return rc ? rc : frame_summary(v, label2id, n_pol, max_len, start3, sum3, mid3, rg2, n_sl, contacts);
 }
 
 // ---- frame summary from a static scratch array (no allocation) ----
@@ -1122,9 +1174,11 @@ static size_t frame_scratch_layout(uintptr_t base, int n_pol, int max_len, Frame
 static int tuv_text(elp_engine* e, int kind, const int32_t* label2id, int n_pol, int max_len,
                     int header_max_pol_length, char* text, uint64_t cap, uint64_t* len) {
     if (!len) return fail(ELP_E_ARG, "len is NULL");
-    ChainBufs B;
-    int rc = chains_run(e, kind, label2id, n_pol, max_len, B);
+    LatView v;
+    int rc = engine_view(e, kind, &v);
     if (rc) return rc;
+    ChainBufs B;
+    if ((rc = chains_run(v, label2id, n_pol, max_len, B))) return rc;
     std::vector<int32_t> ll(n_pol);
     HIPCHK(hipMemcpyAsync(ll.data(), B.line_len.p, sizeof(int32_t) * n_pol, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1193,7 +1247,7 @@ int elp_ring_frame_summary_noalloc(elp_engine* e, const int32_t* label2id, int n
     if (n_max > elp::kSummaryMaxLen)
         return fail(ELP_E_ARG, "frame summary supports rings of up to %d monomers (found %d)", elp::kSummaryMaxLen,
                     n_max);
-    rc = elp::chains_launch_summary(S.bonds, S.lens, n_pol, max_len, n_max, S.sum3, S.mid3, S.contacts, st);
+    rc = elp::chains_launch_summary(S.bonds, S.lens, n_pol, max_len, n_max, 0, S.sum3, S.mid3, S.contacts, st);
     if (rc) return fail(ELP_E_HIP, "summary kernel: %s", hipGetErrorString((hipError_t)rc));
     HIPCHK(hipMemcpyAsync(start3, S.start3, sizeof(int32_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(sum3, S.sum3, sizeof(int64_t) * 3 * n_pol, hipMemcpyDeviceToHost, st));

@@ -111,6 +111,8 @@ def load_elp():
         lib.elp_ring_chain_stats.argtypes = [P, P, I, I, P, P]
         lib.elp_ring_frame_summary.argtypes = [P, P, I, I, P, P, P, P, P, P]
         lib.elp_ring_frame_summary_noalloc.argtypes = [P, P, I, I, P, P, P, P, P, P]
+        lib.elp_lin_frame_summary.argtypes = [P, P, I, I, P, P, P, P, P, P]
+        lib.elp_batch_frame_summary.argtypes = [P, I, P, I, I, P, P, P, P, P, P]
         lib.elp_lin_chains.argtypes = [P, P, I, I, P, P, P]
         lib.elp_lin_tuv.argtypes = [P, P, I, I, I, C.c_char_p, U64, C.POINTER(U64)]
         lib.elp_net_create.argtypes = [C.POINTER(P), I, U32]
@@ -165,6 +167,18 @@ def bench_copy(nbytes, reps=20, device=0):
     ms = C.c_float(0)
     _check(load_elp().elp_bench_copy(device, int(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def _frame_summary(lib, fn, lead, host):
+    """Call one of the elp_*_frame_summary entry points (leading arguments `lead`) for the chains of `host`."""
+    l2i = host.label2id()
+    n = host.n_pol
+    out = {"start3": np.zeros((n, 3), np.int32), "sum3": np.zeros((n, 3), np.int64),
+           "mid3": np.zeros((n, 3), np.int32), "rg2": np.zeros(n), "nsl": np.zeros(n, np.int32),
+           "contacts": np.zeros(host.N, np.uint64)}
+    _check(fn(*lead, l2i.ctypes.data, n, host.N,
+              *(out[k].ctypes.data for k in ("start3", "sum3", "mid3", "rg2", "nsl", "contacts"))), lib)
+    return out
 
 
 class Engine:
@@ -263,15 +277,12 @@ class Engine:
         monomer positions and its monomer N/2 relative to the start, Rg^2 and the stored-length bond count;
         contacts[g] = monomer pairs g apart that touch, summed over rings.  Feed to stats.Series(summary=True).
         noalloc: use elp_ring_frame_summary_noalloc (device buffers from the library's static scratch)."""
-        l2i = host.label2id()
-        n = host.n_pol
-        out = {"start3": np.zeros((n, 3), np.int32), "sum3": np.zeros((n, 3), np.int64),
-               "mid3": np.zeros((n, 3), np.int32), "rg2": np.zeros(n), "nsl": np.zeros(n, np.int32),
-               "contacts": np.zeros(host.N, np.uint64)}
         fn = self.lib.elp_ring_frame_summary_noalloc if noalloc else self.lib.elp_ring_frame_summary
-        _check(fn(self.h, l2i.ctypes.data, n, host.N,
-                  *(out[k].ctypes.data for k in ("start3", "sum3", "mid3", "rg2", "nsl", "contacts"))), self.lib)
-        return out
+        return _frame_summary(self.lib, fn, (self.h,), host)
+
+    def lin_frame_summary(self, host):
+        """ring_frame_summary for a linear-chain engine (middle monomer (N-1)/2)."""
+        return _frame_summary(self.lib, self.lib.elp_lin_frame_summary, (self.h,), host)
 
     def ring_tuv(self, host):
         """The tuv file text of the device lattice (bytes), from the device-side recovery."""
@@ -351,6 +362,11 @@ class Batch:
 
     def sync(self):
         _check(self.lib.elp_batch_sync(self.h), self.lib)
+
+    def frame_summary(self, rep, host):
+        """Frame summary (as Engine.ring_frame_summary) of replica `rep`, read in place on the device;
+        `host` holds that replica's label table and sizes."""
+        return _frame_summary(self.lib, self.lib.elp_batch_frame_summary, (self.h, int(rep)), host)
 
     def last_run_ms(self):
         ms = C.c_float(0)

@@ -165,3 +165,54 @@ def test_device_lin_recovery_rejects_a_damaged_lattice():
     with pytest.raises(elp.ElpError, match="known polymer id"):
         eng.lin_tuv(h)
     eng.close()
+
+
+def _host_summary(ring, h, lat):
+    from elp.stats import Series
+    h.set_lattice(lat)
+    full = Series(ring, h.L, h.n_pol, h.N, 1)
+    full.add_host(h)
+    return full.summary(0)
+
+
+def _same_summary(dev, want):
+    for k in ("start3", "sum3", "mid3", "nsl", "contacts"):
+        assert np.array_equal(dev[k], want[k]), k
+    assert np.allclose(dev["rg2"], want["rg2"], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("N,L,seed,n_time,init", [(100, 32, 3, 50, "reference"), (500, 128, 6, 30, "saw")])
+def test_device_lin_frame_summary_equals_host_reduction(N, L, seed, n_time, init):
+    """elp_lin_frame_summary equals the host reduction of the same frame (middle
+    monomer (N-1)/2, N-1 stored-length candidates, contacts over all pairs)."""
+    _say(f"[linear frame summary] N={N} L={L} start={init}")
+    h, eng = _run_lin(N, L, seed, n_time, init)
+    dev = eng.lin_frame_summary(h)
+    _same_summary(dev, _host_summary(False, h, eng.download()))
+    # consecutive monomers always touch: N - 1 pairs one apart per chain
+    assert dev["contacts"][1] == h.n_pol * (N - 1)
+    eng.close()
+
+
+@pytest.mark.parametrize("kind", ["ring", "linear"])
+def test_batch_frame_summary_equals_host_reduction(kind):
+    """elp_batch_frame_summary reads each replica in place: equal to the host
+    reduction of that replica's download, for replicas of different boxes."""
+    _say(f"[batch frame summary] {kind}")
+    ring = kind == "ring"
+    cls = elp.RingHost if ring else elp.LinHost
+    tile = (16, 16, 16)
+    hosts = [cls(N=n, L=l, density=1.2, seed=s, box_unit=16) for n, l, s in ((100, 32, 1), (60, 48, 2), (100, 32, 3))]
+    bat = elp.Batch(kind, [h.L for h in hosts], tile, elp.SUBLAT_DENSE, 13 if ring else 16)
+    for r, h in enumerate(hosts):
+        bat.upload(r, h.lattice())
+        if ring:
+            h.load_shift(tile)
+    org = np.stack([h.origins(tile, 30) for h in hosts], axis=1)
+    bat.run(org, tau0=0, seeds=[h.seed for h in hosts], streams=[0] * len(hosts))
+    for r, h in enumerate(hosts):
+        dev = bat.frame_summary(r, h)
+        _same_summary(dev, _host_summary(ring, h, bat.download(r)))
+    with pytest.raises(elp.ElpError, match="out of range"):
+        bat.frame_summary(len(hosts), hosts[0])
+    bat.close()

@@ -330,3 +330,52 @@ def test_trained_network_raises_accepted_moves_per_attempt():
     trained.close()
     uniform.close()
     eng.close()
+
+
+def test_learned_run_at_c5_size():
+    """The learned-proposal configuration at its full size: ring melt L=128,
+    N=4000, density 1.2 (629 rings) from the compact start, with a network
+    trained on 10^6 moves harvested from the plain engine on the L=96, N=1000
+    melt.  Over 20 learned time units: one proposal per slot and substep is
+    counted, some but not all are accepted, every ring stays intact with its
+    label sequence and is recovered by the host, the lattice moves, and the
+    features of sampled full-size patches equal their definition."""
+    from ringcheck import ring_label_sequences, site_invariants
+    _say("[learned C5] harvest on L=96 N=1000")
+    tile = (16, 16, 16)
+    h2 = elp.RingHost(N=1000, L=96, density=1.2, seed=0)
+    e2 = elp.Engine("ring", h2.L, tile, elp.SUBLAT_DENSE, 13)
+    e2.set_rules(elp.RULES_BALANCED)
+    e2.upload(h2.lattice())
+    h2.load_shift(tile)
+    e2.run(h2.origins(tile, 300), tau0=0, seed=h2.seed)
+    net = elp.Net(seed=31)
+    net.harvest(e2, 1_000_000, seed=37)
+    loss = net.train(epochs=2, batch=4096, lr=3e-3)
+    e2.close()
+    _say(f"  trained: last-epoch loss {loss:.3f}")
+    _say("[learned C5] L=128 N=4000, 20 learned time units")
+    h = elp.RingHost(N=4000, L=128, density=1.2, seed=0)
+    assert h.n_pol == 629
+    lat0 = h.lattice()
+    before = ring_label_sequences(lat0, h.L)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_CUBIC, 51)
+    eng.set_rules(elp.RULES_BALANCED)
+    eng.upload(lat0)
+    h.load_shift(tile)
+    n_time = 20
+    prop, acc = net.run(eng, h.origins(tile, n_time), tau0=0, seed=h.seed)
+    ntile = (128 // 16) ** 3
+    assert prop == n_time * ntile * 64 * 4 * 51
+    assert 0 < acc < prop
+    got = eng.download()
+    _say(f"  {acc} of {prop} proposals accepted; {np.count_nonzero(got != lat0)} sites changed")
+    assert np.count_nonzero(got != lat0) > 10000
+    assert site_invariants(got, h.L) == 629 * 4000
+    assert ring_label_sequences(got, h.L) == before
+    h.set_lattice(got)                                   # host recovery of every labelled ring
+    rng = np.random.default_rng(12)
+    P = patches_of(got, h.L, rng.choice(np.flatnonzero(got), 3000, replace=False))
+    assert np.array_equal(elp.patch_features(P), np.stack([features_of(p) for p in P]))
+    eng.close()
+    net.close()

@@ -50,16 +50,16 @@ def _run(name):
     series = []
     for r, h in enumerate(hosts):
         bat.upload(r, h.lattice())
-        series.append(Series(False, h.L, h.n_pol, h.N, T // INTERVAL + 1))
-        series[r].add_host(h)
+        # frames reduced on the device, replica by replica in place: no lattice download per frame
+        series.append(Series(False, h.L, h.n_pol, h.N, T // INTERVAL + 1, summary=True))
+        series[r].add_summary(bat.frame_summary(r, h))
     tau = 0
     for k in range(T // INTERVAL):
         org = np.stack([h.origins(tile, INTERVAL) for h in hosts], axis=1)
         bat.run(org, tau0=tau, seeds=[h.seed for h in hosts], streams=[0] * 8, units_per_graph=16)
         tau += INTERVAL
         for r, h in enumerate(hosts):
-            h.set_lattice(bat.download(r))
-            series[r].add_host(h)
+            series[r].add_summary(bat.frame_summary(r, h))
         if k % 10 == 9:
             _say(f"  {tau} time units")
     res = []

@@ -15,7 +15,12 @@ on a site (its equilibrium has about 10 % of bonds as stored length following
 stored length), while a GPU site holds at most two, and the GPU move set is
 not reversible (tests/test_ring_balance.py).  The test therefore pins the
 structural difference and reports the efpol comparison instead of asserting
-agreement.
+agreement.  efpol restricted to at most two monomers per site
+(tests/golden/efpol_stats/ring2_L32_N100.json, from the GPU engine's own
+starts) samples the uniform measure over the GPU state space (efpol's
+sl_equal table is symmetric: tests/test_efpol_table.py), which is the
+equilibrium of the balanced rates: there the statics must agree within
+3 sigma, for both schedules.
 """
 import json
 import os
@@ -31,6 +36,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = json.load(open(os.path.join(HERE, "golden", "efpol_stats", "ring_L32_N100.json")))
+REF2 = json.load(open(os.path.join(HERE, "golden", "efpol_stats", "ring2_L32_N100.json")))
 L, N, T, INTERVAL, N_THERM, LAGS = REF["L"], REF["N"], REF["time_units"], REF["interval"], REF["n_therm_frames"], REF["lags"]
 S_POINTS = [2, 3, 5, 10, 20, 50]
 SCHEDULES = {"reference 32^3 cubic 51": ((32, 32, 32), elp.SUBLAT_CUBIC, 51),
@@ -58,18 +64,19 @@ def _run(name, rules=elp.RULES_REFERENCE):
     for r, h in enumerate(hosts):
         bat.upload(r, h.lattice())
         h.load_shift(tile)
-        series.append(Series(True, h.L, h.n_pol, h.N, T // INTERVAL + 1))
-        series[r].add_host(h)
+        # frames reduced on the device, replica by replica in place: no lattice download per frame
+        series.append(Series(True, h.L, h.n_pol, h.N, T // INTERVAL + 1, summary=True))
+        series[r].add_summary(bat.frame_summary(r, h))
     tau = 0
     for _ in range(T // INTERVAL):
         org = np.stack([h.origins(tile, INTERVAL) for h in hosts], axis=1)
         bat.run(org, tau0=tau, seeds=[h.seed for h in hosts], streams=[0] * 8, units_per_graph=16)
         tau += INTERVAL
         for r, h in enumerate(hosts):
-            h.set_lattice(bat.download(r))
-            series[r].add_host(h)
+            series[r].add_summary(bat.frame_summary(r, h))
     res = []
-    for h, ser in zip(hosts, series):
+    for r, (h, ser) in enumerate(zip(hosts, series)):
+        h.set_lattice(bat.download(r))             # the last frame, for the stacking count
         stack3 = 0
         for c in range(h.n_pol):
             b = h.polymer(c)[1]
@@ -150,3 +157,28 @@ def test_efpol_comparison_model_difference(rules):
         rows.append(f"g3 at lags {LAGS} x {INTERVAL}: GPU {np.round(g3g, 3)} efpol {np.round(g3e, 3)}")
         _say(f"[efpol comparison] {name} rules {rules}\n  " + "\n  ".join(rows))
     assert np.mean([r["stack3"] for r in ef]) > 0.05
+
+
+@pytest.mark.parametrize("name", list(SCHEDULES))
+def test_balanced_rules_equal_efpol_with_at_most_two_per_site(name):
+    """Balanced rates sample the uniform measure over GPU lattice states; so
+    does efpol restricted to at most two monomers per site.  Rg^2, the
+    stored-length ratio and P(s) agree within 3 sigma.  Reported beside it:
+    the stacking effect (efpol vs efpol-2) and the rates effect (GPU
+    reference vs balanced rates) that make up the efpol comparison."""
+    g, e2, ef = _run(name, elp.RULES_BALANCED), REF2["seeds"], REF["seeds"]
+    gr = _run(name, elp.RULES_REFERENCE)
+    assert all(r["stack3"] == 0 for r in e2)
+    zs = {}
+    for key in ("rg2", "slrat"):
+        zs[key] = _z([r[key] for r in g], [r[key] for r in e2])
+        (m_g, s_g), (m_2, s_2) = _mean_se([r[key] for r in g]), _mean_se([r[key] for r in e2])
+        m_e, m_r = _mean_se([r[key] for r in ef])[0], _mean_se([r[key] for r in gr])[0]
+        _say(f"[efpol-2] {name} {key}: GPU balanced {m_g:.5f}+-{s_g:.5f} efpol-2 {m_2:.5f}+-{s_2:.5f} "
+             f"z {zs[key]:.2f}; stacking (efpol - efpol-2) {m_e - m_2:+.4f}, rates (GPU reference - balanced) "
+             f"{m_r - m_g:+.4f}")
+    zs["pc"] = _z([r["pc"][S_POINTS] for r in g], [np.asarray(r["pc"])[S_POINTS] for r in e2])
+    _say(f"[efpol-2] {name} P(s) at s={S_POINTS}: z = {np.round(zs['pc'], 2)}")
+    g3g, g3e = _mean_se([r["g3"] for r in g])[0], _mean_se([r["g3"] for r in e2])[0]
+    _say(f"[efpol-2] {name} g3 at lags {LAGS} x {INTERVAL}: GPU {np.round(g3g, 3)} efpol-2 {np.round(g3e, 3)}")
+    assert abs(zs["rg2"]) < 3 and abs(zs["slrat"]) < 3 and np.all(np.abs(zs["pc"]) < 3), zs
