@@ -83,6 +83,19 @@ struct elp_engine {
     uint64_t attempts = 0;
 };
 
+// Move counters: elp::kCountSlots slots of elp::kCountStride words; the
+// totals (empty, face, accepted) are the sums over the slots.
+constexpr size_t kCountBytes = sizeof(unsigned long long) * elp::kCountSlots * elp::kCountStride;
+static hipError_t sum_counts(const unsigned long long* d, hipStream_t st, unsigned long long out[3]) {
+    std::vector<unsigned long long> h((size_t)elp::kCountSlots * elp::kCountStride);
+    hipError_t e = hipMemcpyAsync(h.data(), d, kCountBytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    out[0] = out[1] = out[2] = 0;
+    for (int i = 0; i < elp::kCountSlots; i++)
+        for (int k = 0; k < 3; k++) out[k] += h[(size_t)i * elp::kCountStride + k];
+    return e;
+}
+
 // move attempts of one time unit: every slot of every tile in every substep
 static uint64_t attempts_per_unit(int kind, int LT, int LU, int LV, int tt, int tu, int tv, int sublattice,
                                   int n_iter) {
@@ -246,8 +259,8 @@ int elp_count_moves(elp_engine* e, int enable) {
     if (!e) return fail(ELP_E_ARG, "NULL engine");
     HIPCHK(hipSetDevice(e->device));
     if (enable) {
-        if (!e->d_counts) HIPCHK(hipMalloc(&e->d_counts, 3 * sizeof(unsigned long long)));
-        HIPCHK(hipMemsetAsync(e->d_counts, 0, 3 * sizeof(unsigned long long), e->stream));
+        if (!e->d_counts) HIPCHK(hipMalloc(&e->d_counts, kCountBytes));
+        HIPCHK(hipMemsetAsync(e->d_counts, 0, kCountBytes, e->stream));
         e->attempts = 0;
     }
     e->counting = enable != 0;
@@ -258,8 +271,8 @@ int elp_move_counts(elp_engine* e, uint64_t counts[4]) {
     if (!e || !counts) return fail(ELP_E_ARG, "NULL argument");
     if (!e->d_counts) return fail(ELP_E_STATE, "move counting was never enabled");
     HIPCHK(hipSetDevice(e->device));
-    unsigned long long c[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(c, e->d_counts, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    unsigned long long c[3];
+    HIPCHK(sum_counts(e->d_counts, e->stream, c));
     HIPCHK(hipStreamSynchronize(e->stream));
     counts[0] = e->attempts;
     counts[1] = c[0]; counts[2] = c[1]; counts[3] = c[2];
@@ -514,8 +527,8 @@ int elp_batch_count_moves(elp_batch* b, int enable) {
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
     if (enable) {
-        if (!b->d_counts) HIPCHK(hipMalloc(&b->d_counts, 3 * sizeof(unsigned long long)));
-        HIPCHK(hipMemsetAsync(b->d_counts, 0, 3 * sizeof(unsigned long long), b->stream));
+        if (!b->d_counts) HIPCHK(hipMalloc(&b->d_counts, kCountBytes));
+        HIPCHK(hipMemsetAsync(b->d_counts, 0, kCountBytes, b->stream));
         b->attempts = 0;
     }
     if ((enable != 0) != b->counting && b->gexec) {   // the cached graph holds the other kernel
@@ -531,8 +544,8 @@ int elp_batch_move_counts(elp_batch* b, uint64_t counts[4]) {
     if (!b || !counts) return fail(ELP_E_ARG, "NULL argument");
     if (!b->d_counts) return fail(ELP_E_STATE, "move counting was never enabled");
     HIPCHK(hipSetDevice(b->device));
-    unsigned long long c[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(c, b->d_counts, sizeof c, hipMemcpyDeviceToHost, b->stream));
+    unsigned long long c[3];
+    HIPCHK(sum_counts(b->d_counts, b->stream, c));
     HIPCHK(hipStreamSynchronize(b->stream));
     counts[0] = b->attempts;
     counts[1] = c[0]; counts[2] = c[1]; counts[3] = c[2];

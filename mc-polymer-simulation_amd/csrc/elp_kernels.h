@@ -57,11 +57,21 @@ struct BatchLaunch {
 // gates); the move is rejected otherwise; the move is made.
 enum : uint32_t { kMoveEmpty = 0, kMoveFace = 1, kMoveRejected = 2, kMoveAccepted = 3 };
 
+// Move counters of counting launches: kCountSlots slots of kCountStride
+// words (one 128-byte line each); workgroup b adds to slot b % kCountSlots,
+// so that the atomics of a launch spread over many lines instead of
+// serialising on one (a single line cost about 12 ns per wave atomic).
+// Word k of a slot: 0 empty, 1 face, 2 accepted; the totals are the sums
+// over the slots.
+constexpr int kCountSlots = 512;
+constexpr int kCountStride = 16;
+
 // Launch one time unit (nblocks = number of tiles).  probe=true only checks
-// that the schedule is compiled in.  A non-null p.counts launches the
-// counting instantiation of the kernel, which adds the outcomes of its
-// attempts to counts[0..2] (empty, face, accepted) and moves exactly as the
-// plain one.  Returns 0, a hipError_t, or -1 when the schedule is not available.
+// that the schedule is compiled in.  A non-null p.counts (kCountSlots x
+// kCountStride words) launches the counting instantiation of the kernel,
+// which adds the outcomes of its attempts to its slot (empty, face, accepted)
+// and moves exactly as the plain one.  Returns 0, a hipError_t, or -1 when
+// the schedule is not available.
 int ring_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,
                 uint8_t* glat, const RingLaunch& p);
 int lin_launch(int tt, int tu, int tv, bool dense, int threads, bool probe, int nblocks, hipStream_t stream,

@@ -196,14 +196,16 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     asm volatile("" : "+v"(tidStore));
     tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tidStore);
     if constexpr (COUNT) {
-        // one sum per wave, added by its first lane (vector atomics)
+        // one sum per wave, added by its first lane (vector atomics) to the
+        // workgroup's counter slot
         nEmpty = wave_sum(nEmpty);
         nFace = wave_sum(nFace);
         nAcc = wave_sum(nAcc);
         if ((tid & 63) == 0) {
-            atomicAdd(&p.counts[0], (unsigned long long)nEmpty);
-            atomicAdd(&p.counts[1], (unsigned long long)nFace);
-            atomicAdd(&p.counts[2], (unsigned long long)nAcc);
+            unsigned long long* c = p.counts + (size_t)(blockIdx.x % kCountSlots) * kCountStride;
+            atomicAdd(&c[0], (unsigned long long)nEmpty);
+            atomicAdd(&c[1], (unsigned long long)nFace);
+            atomicAdd(&c[2], (unsigned long long)nAcc);
         }
     }
 }
