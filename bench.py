@@ -223,8 +223,9 @@ def bench_batch(args, tile):
 
 
 def _rg2_now(host, eng):
-    """Mean Rg^2 of the engine's rings, reduced on the device (no lattice download)."""
-    return float(np.mean(eng.ring_frame_summary(host)["rg2"]))
+    """Mean Rg^2 of the engine's rings from the device-side chain walk (no lattice download, no
+    contact histogram)."""
+    return float(np.mean(eng.ring_stats(host)[0]))
 
 
 def bench_learned(args):

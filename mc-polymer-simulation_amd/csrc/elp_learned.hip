@@ -261,24 +261,108 @@ __device__ void lp_softmax(const float* z, float* p) {
     for (int m = 0; m < LP_NMOVE; m++) p[m] *= inv;
 }
 
+// ------------------------------------------------ inference forward pass
+// The inference kernels (lp_forward_kernel, lp_ring_time_unit) keep every
+// weight operand in registers for the whole launch: wave w computes columns
+// 16w..16w+15 of layers 1 and 2 for all 64 rows (the same B fragments for
+// its four row blocks) and rows 16w..16w+15 of layer 3 (all 16 logits).
+// Lane l holds B(4(l/16)+j, l%16), j < 4, of every k-step (17 bf16x4 + two
+// biases: 36 VGPRs).  The A operands (activation rows) are read from LDS as
+// one 8-byte read per lane and k-step; row strides of 148 and 68 bf16 (74 and
+// 34 dwords) put the 16 rows of a tile on distinct even banks.
+constexpr int LP_XS = LP_NIN + 4;
+constexpr int LP_HS = LP_NH + 4;
+struct LpAct {
+    uint16_t X[64][LP_XS];
+    uint16_t H1[64][LP_HS];
+    uint16_t H2[64][LP_HS];
+    float Z[64][LP_NOUT];
+};
+struct LpFrag {
+    bf16x4 w1[LP_NIN / 16];
+    bf16x4 w2[LP_NH / 16];
+    bf16x4 w3[LP_NH / 16];
+    float b2, b3;
+};
+
+__device__ __forceinline__ void lp_load_frag(LpFrag& F, const float* __restrict__ prm) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int kk = 0; kk < LP_NIN / 16; kk++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) F.w1[kk][j] = (short)to_bf16(prm[LP_OFF_W1 + (16 * kk + kq + j) * LP_NH + 16 * w + c]);
+#pragma unroll
+    for (int kk = 0; kk < LP_NH / 16; kk++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            F.w2[kk][j] = (short)to_bf16(prm[LP_OFF_W2 + (16 * kk + kq + j) * LP_NH + 16 * w + c]);
+            F.w3[kk][j] = (short)to_bf16(prm[LP_OFF_W3 + (16 * kk + kq + j) * LP_NOUT + c]);
+        }
+    F.b2 = prm[LP_OFF_B2 + 16 * w + c];
+    F.b3 = prm[LP_OFF_B3 + c];
+}
+
+__device__ __forceinline__ bf16x4 lds_bf16x4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+// Forward pass of the 64 rows of A.X into A.Z; all 4 waves of the workgroup
+// (blockDim = 256) call it; ends with a barrier.
+__device__ void lp_forward_reg(LpAct& A, const LpFrag& F) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {                   // layer 1 (bias through the constant feature)
+        const uint16_t* a = &A.X[16 * rb + c][kq];
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < LP_NIN / 16; kk++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(a + 16 * kk), F.w1[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) A.H1[16 * rb + kq + i][16 * w + c] = to_bf16(fmaxf(acc[i], 0.f));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {                   // layer 2
+        const uint16_t* a = &A.H1[16 * rb + c][kq];
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < LP_NH / 16; kk++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(a + 16 * kk), F.w2[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) A.H2[16 * rb + kq + i][16 * w + c] = to_bf16(fmaxf(acc[i] + F.b2, 0.f));
+    }
+    __syncthreads();
+    {                                                  // layer 3: row block w
+        const uint16_t* a = &A.H2[16 * w + c][kq];
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < LP_NH / 16; kk++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(a + 16 * kk), F.w3[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) A.Z[16 * w + kq + i][c] = acc[i] + F.b3;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------- kernels
 // Probabilities of n feature rows (tests and diagnostics).
 __global__ __launch_bounds__(256) void lp_forward_kernel(const float* __restrict__ prm, const int8_t* __restrict__ x,
                                                           int n, float* __restrict__ prob) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    LpShared& S = *reinterpret_cast<LpShared*>(lds_raw);
-    lp_load_net(S, prm);
+    LpAct& A = *reinterpret_cast<LpAct*>(lds_raw);
+    LpFrag F;
+    lp_load_frag(F, prm);
     for (int b0 = blockIdx.x * 64; b0 < n; b0 += gridDim.x * 64) {
         __syncthreads();
         for (int i = threadIdx.x; i < 64 * LP_NIN; i += blockDim.x) {
             const int r = i / LP_NIN, c = i % LP_NIN;
-            S.X[r][c] = (b0 + r < n) ? to_bf16((float)x[(size_t)(b0 + r) * LP_NIN + c]) : 0;
+            A.X[r][c] = (b0 + r < n) ? to_bf16((float)x[(size_t)(b0 + r) * LP_NIN + c]) : 0;
         }
         __syncthreads();
-        lp_forward(S);
+        lp_forward_reg(A, F);
         if (threadIdx.x < 64 && b0 + threadIdx.x < n) {
             float p[LP_NMOVE];
-            lp_softmax(S.Z[threadIdx.x], p);
+            lp_softmax(A.Z[threadIdx.x], p);
             for (int m = 0; m < LP_NMOVE; m++) prob[(size_t)(b0 + threadIdx.x) * LP_NMOVE + m] = p[m];
         }
     }
@@ -523,7 +607,8 @@ __global__ void lp_patch_features_kernel(const uint8_t* __restrict__ in, int n, 
 // counter (tau, tile, slot | 0x40000000, iter) words 2s, 2s+1 = (u_move, u_accept)
 // for s < 2 from the first call and s >= 2 from counter word 3 + 1.
 struct LpMcShared {
-    LpShared F;
+    LpAct A;
+    uint32_t poff[128];               // patch entry i -> (i%5) | (i/5%5) << 8 | (i/25) << 16
     uint8_t tile[4096];
     uint8_t qS[64][128];
     float pS[64][LP_NMOVE + 1];
@@ -535,7 +620,7 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
                                                           const float* __restrict__ prm, unsigned long long* __restrict__ stats) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     LpMcShared& M = *reinterpret_cast<LpMcShared*>(lds_raw);
-    LpShared& S = M.F;
+    LpAct& S = M.A;
     constexpr int TT = 16;
     const int tid = threadIdx.x;
     const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
@@ -543,7 +628,9 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
     const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
     const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
     const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TT, gv0 = p.ov + kv * TT;
-    lp_load_net(S, prm);
+    LpFrag F;
+    lp_load_frag(F, prm);
+    if (tid < 128) M.poff[tid] = (uint32_t)(tid % 5) | (uint32_t)(tid / 5 % 5) << 8 | (uint32_t)(tid / 25) << 16;
     for (int i = tid; i < 4096; i += 256) {
         int gt = gt0 + (i & 15), gu = gu0 + ((i >> 4) & 15), gv = gv0 + (i >> 8);
         gt -= (gt >= p.LT) ? p.LT : 0;
@@ -564,21 +651,25 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
         }
         for (int s = 0; s < 4; s++) {
             const uint32_t c = (offw >> (8 * s)) & 0x3fu;
-            // patches of all 64 slots, by all threads: slot e / 125, entry e % 125
-            for (int e = tid; e < 64 * 125; e += 256) {
-                const int sl = e / 125, i = e - 125 * sl;
-                const int a = 4 * (sl & 3) + (int)(c & 3u) + i % 5 - 2;
-                const int b = 4 * ((sl >> 2) & 3) + (int)((c >> 2) & 3u) + (i / 5) % 5 - 2;
-                const int d = 4 * (sl >> 4) + (int)((c >> 4) & 3u) + i / 25 - 2;
-                const uint8_t v = ((unsigned)a < 16u && (unsigned)b < 16u && (unsigned)d < 16u)
-                                      ? M.tile[a + 16 * (b + 16 * d)] : WALL;
-                M.qS[sl][i] = v;
-                S.X[sl][i] = v ? BF_ONE : 0;
+            // patches of all 64 slots, by all threads: four threads per slot, entries
+            // part, part+4, ... of its 5^3 patch (offsets from the LDS table)
+            {
+                const int sl = tid >> 2, part = tid & 3;
+                const int a0 = 4 * (sl & 3) + (int)(c & 3u) - 2, b0 = 4 * ((sl >> 2) & 3) + (int)((c >> 2) & 3u) - 2,
+                          d0 = 4 * (sl >> 4) + (int)((c >> 4) & 3u) - 2;
+                for (int i = part; i < 125; i += 4) {
+                    const uint32_t o = M.poff[i];
+                    const int a = a0 + (int)(o & 0xffu), b = b0 + (int)((o >> 8) & 0xffu), d = d0 + (int)(o >> 16);
+                    const uint8_t v = ((unsigned)a < 16u && (unsigned)b < 16u && (unsigned)d < 16u)
+                                          ? M.tile[a + 16 * (b + 16 * d)] : WALL;
+                    M.qS[sl][i] = v;
+                    S.X[sl][i] = v ? BF_ONE : 0;
+                }
             }
             __syncthreads();
             if (tid < 64) lp_features_tail(M.qS[tid], S.X[tid]);
             __syncthreads();
-            lp_forward(S);
+            lp_forward_reg(S, F);
             // propose m ~ q(. | S), apply it to the patch, features of S'
             if (tid < 64) {
                 lp_softmax(S.Z[tid], M.pS[tid]);
@@ -606,7 +697,7 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
                 lp_features_tail(M.qS[tid], S.X[tid]);
             }
             __syncthreads();
-            lp_forward(S);
+            lp_forward_reg(S, F);
             // accept with min(1, q(rev | S') / q(m | S)); write the changed sites
             if (tid < 64) {
                 nprop++;
@@ -649,7 +740,7 @@ static int set_lds(K kernel, size_t lds) {
 }
 
 int lp_launch_forward(const float* prm, const int8_t* x, int n, float* prob, hipStream_t st) {
-    const size_t lds = sizeof(LpShared);
+    const size_t lds = sizeof(LpAct);
     int rc = set_lds(lp_forward_kernel, lds);
     if (rc) return rc;
     const int blocks = n / 64 + 1 < 1024 ? n / 64 + 1 : 1024;
