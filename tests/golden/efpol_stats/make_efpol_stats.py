@@ -12,8 +12,10 @@ to ring_L32_N100.json or lin_L32_N100.json next to this script.
 Mode ring2 runs efpol's dynamics restricted to at most two monomers per site
 (`efpol_port_ring from START SEED DIR T INTERVAL 2`: every move that would
 stack a third monomer is rejected), from the GPU engine's own start (the
-RingHost melt of seed k, k = 0..7, efpol seed 2k+2), 60000 time units with a
-tuv file every 1000, the first 21 frames discarded, and writes
+RingHost melt of seed k, k = 0..31, efpol seed 2k+2), 200000 time units with a
+tuv file every 1000, the first 101 frames discarded (ring Rg^2 from the compact
+start relaxes over several 10^4 units: a ring's centre of mass moves about one
+radius of gyration in 3*10^4), and writes
 ring2_L32_N100.json.  efpol's sl_equal transition table is symmetric, so this
 chain samples the uniform measure over the states a GPU lattice holds: the
 equilibrium of the GPU engine with balanced rates.
@@ -55,7 +57,7 @@ def stacking(files):
     return n3 / tot
 
 
-T2, N_THERM2 = 60000, 21
+T2, N_THERM2, NSEED2 = 200000, 101, 32
 
 
 def frames_of(d, t_units, interval):
@@ -78,22 +80,23 @@ def seed_stats(ring, files, n_therm):
 
 def main_ring2(work):
     exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
-    procs, dirs = [], []
-    for k in range(8):
+    dirs = []
+    for k in range(NSEED2):
         d = os.path.join(work, f"k{k}")
         os.makedirs(d, exist_ok=True)
         h = elp.RingHost(N=N, L=L, density=1.2, seed=k)
-        start = os.path.join(d, "start.res")
-        h.write_tuv(start)
+        h.write_tuv(os.path.join(d, "start.res"))
         h.close()
         dirs.append(d)
-        procs.append(subprocess.Popen([exe, "from", start, str(2 * k + 2), d, str(T2), str(INTERVAL), "2"],
-                                      stdout=subprocess.DEVNULL))
-    for p in procs:
-        assert p.wait() == 0
+    for b in range(0, NSEED2, 8):                 # 8 processes at a time
+        procs = [subprocess.Popen([exe, "from", os.path.join(dirs[k], "start.res"), str(2 * k + 2), dirs[k], str(T2),
+                                   str(INTERVAL), "2"], stdout=subprocess.DEVNULL) for k in range(b, min(b + 8, NSEED2))]
+        for p in procs:
+            assert p.wait() == 0
     out = {"L": L, "N": N, "density": 1.2, "model": "sl_equal, at most two monomers per site", "time_units": T2,
            "interval": INTERVAL, "n_therm_frames": N_THERM2, "lags": LAGS,
-           "start": "RingHost(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2", "seeds": []}
+           "start": f"RingHost(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2, k = 0..{NSEED2 - 1}",
+           "seeds": []}
     for k, d in enumerate(dirs):
         row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(True, frames_of(d, T2, INTERVAL), N_THERM2)}
         assert row["stack3"] == 0

@@ -189,3 +189,54 @@ def test_summary_series_equals_full_series_on_a_melt():
     sm = full.summary(3)
     assert np.array_equal(s.rg2(3), full.rg2(3))
     assert sm["contacts"].sum() > 0 and sm["nsl"].sum() > 0
+
+
+def test_msd_drift_convention_of_the_reference_differs_negligibly():
+    """The reference analysis subtracts the drift of the system's mean in mixed
+    units: avg_pos.dat holds the mean over chains of cms/sqrt(2)
+    (analysis/mode_compute.c:244-251, mode_io.c:640-652), which g3 subtracts from
+    a centre-of-mass displacement in bond units (:548-566) and g1 from a
+    middle-monomer displacement in lattice units before dividing by 2 (:506-528).
+    So g3 removes 1/sqrt(2) of the drift and g1 1/2 of it, where this library
+    removes all of it.  For g3 the difference is exactly (1 - 1/sqrt(2))^2 times
+    the mean-square displacement of the system mean (the chains' mean displacement
+    is the system's, so the cross term vanishes); on a replay melt (L=32, N=100,
+    393 rings, lags 40..160 time units from the start) g3 differs by 0.14-0.23 %
+    and g1 by 0.2-1.0 %."""
+    from ringcheck import replay_ring
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=6)
+    tile = (16, 16, 16)
+    h.load_shift(tile)
+    nf = 6
+    full = Series(True, h.L, h.n_pol, h.N, nf)
+    full.add_host(h)
+    lat, tau = h.lattice(), 0
+    for f in range(1, nf):
+        org = h.origins(tile, 40)
+        lat = replay_ring(lat, h.L, tile, elp.SUBLAT_DENSE, 13, org, tau, h.seed)
+        tau += 40
+        h.set_lattice(lat)
+        full.add_host(h)
+    pos = np.array([[xyz(full.positions(f, c).astype(float)) for c in range(h.n_pol)] for f in range(nf)])
+    cms = pos.mean(axis=2)                                  # (frame, chain, 3), bond units
+    mid = pos[:, :, h.N // 2]
+    sysm = cms.mean(axis=1)                                 # system mean per frame
+
+    def msd(x, s):
+        out = []
+        for k in range(nf - 1):
+            d = x[:nf - k] - x[k:] - s * (sysm[:nf - k] - sysm[k:])[:, None, :]
+            out.append(np.mean(np.sum(d * d, axis=2)))
+        return np.array(out)
+    g1, g3 = full.msd(0, nf - 1)
+    assert np.allclose(msd(mid, 1.0), g1, rtol=1e-12) and np.allclose(msd(cms, 1.0), g3, rtol=1e-12)
+    g1_ref, g3_ref = msd(mid, 0.5), msd(cms, 1 / np.sqrt(2))
+    assert g3[1] > 0.05                                     # the chains moved
+    # g3: the chains' mean displacement is the system's, so the cross term vanishes and the
+    # reference's g3 exceeds this library's by exactly (1 - 1/sqrt(2))^2 x the system mean's MSD
+    sys_msd = np.array([np.mean(np.sum((sysm[:nf - k] - sysm[k:]) ** 2, axis=1)) for k in range(nf - 1)])
+    assert np.allclose(g3_ref - g3, (1 - 1 / np.sqrt(2)) ** 2 * sys_msd, rtol=1e-9, atol=1e-15)
+    rel3, rel1 = g3_ref[1:] / g3[1:] - 1, g1_ref[1:] / g1[1:] - 1
+    print(f"reference drift convention: g3 +{np.round(100 * rel3, 3)} %, g1 {np.round(100 * rel1, 3)} % at lags "
+          f"40..160 time units")
+    assert np.all(np.abs(rel3) < 2e-2) and np.all(np.abs(rel1) < 2e-2), (rel3, rel1)
