@@ -27,11 +27,12 @@
  *   efpol_port SEED DIR DENSITY TIME INTERVAL NMONO DBL_STEP L MODEL
  *       same argv as the reference (cpupol/efpol.c:22-38); writes the same
  *       files DIR/N_<NMONO>/t=<T>_dev=0.res and simulation_settings.txt.
- *   efpol_port from TUV SEED DIR TIME INTERVAL MAXSTACK
- *       efpol's sl_equal dynamics from the chains of a tuv file; MAXSTACK 2
- *       rejects every move that would put a third monomer on a site, so the
- *       chain samples the state space of the GPU engines (at most one
- *       stored-length monomer per site); 0 = efpol itself.
+ *   efpol_port from TUV SEED DIR TIME INTERVAL MAXSTACK [MODEL]
+ *       efpol's dynamics (MODEL sl_equal, the default, sl_double or sl_quad)
+ *       from the chains of a tuv file; MAXSTACK 2 rejects every move that
+ *       would put a third monomer on a site, so the chain samples the state
+ *       space of the GPU engines (at most one stored-length monomer per
+ *       site); 0 = efpol itself.
  *   efpol_port table MODEL
  *       prints the transition table, one slot per line.
  *   efpol_port bench L NMONO DENSITY SEED NTIME [NWARM]
@@ -400,17 +401,21 @@ int main(int argc, char** argv) {
         return 0;
     }
     if (argc >= 2 && !strcmp(argv[1], "from")) {
-        /* from TUV SEED DIR TIME INTERVAL MAXSTACK: efpol's sl_equal dynamics from the chains of a tuv
+        /* from TUV SEED DIR TIME INTERVAL MAXSTACK [MODEL]: efpol's dynamics from the chains of a tuv
            file, with at most MAXSTACK (2, or 0 = no limit) monomers per site; writes
            DIR/N_<len>/t=<T>_dev=0.res every INTERVAL time units like the reference driver */
-        if (argc < 8) { fprintf(stderr, "usage: %s from TUV SEED DIR TIME INTERVAL MAXSTACK\n", argv[0]); return 2; }
+        if (argc < 8) {
+            fprintf(stderr, "usage: %s from TUV SEED DIR TIME INTERVAL MAXSTACK [sl_equal|sl_double|sl_quad]\n", argv[0]);
+            return 2;
+        }
         Sim S; memset(&S, 0, sizeof S);
         if (read_tuv(&S, argv[2])) { fprintf(stderr, "cannot read %s\n", argv[2]); return 199; }
         uint32_t seed = (uint32_t)atol(argv[3]);
         long tMax = atol(argv[5]), interval = atol(argv[6]);
         S.maxStack = atoi(argv[7]);
         lfsr113_seed_efpol(S.rng, seed);
-        build_trans(&S, MODEL_EQUAL);
+        const char* mname = argc > 8 ? argv[8] : "sl_equal";
+        build_trans(&S, !strcmp(mname, "sl_double") ? MODEL_DOUBLE : !strcmp(mname, "sl_quad") ? MODEL_QUAD : MODEL_EQUAL);
         char sub[4096], file[8192];
         snprintf(sub, sizeof sub, "%s/N_%i", argv[4], S.polSize);
         mkdir_p(sub);

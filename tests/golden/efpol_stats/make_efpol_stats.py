@@ -20,7 +20,15 @@ ring2_L32_N100.json.  efpol's sl_equal transition table is symmetric, so this
 chain samples the uniform measure over the states a GPU lattice holds: the
 equilibrium of the GPU engine with balanced rates.
 
-    python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear|ring2] [workdir]
+Mode lin2 does the same for linear chains with efpol's sl_double model (from
+the LinHost melt of seed k, k = 0..15, 400000 time units, a tuv file every
+10000, the first 11 frames discarded, as the linear mode) and writes
+lin2_L32_N100.json.  sl_double weights each stored-length bond by 2 in
+detailed balance (tests/test_efpol_table.py), the stationary law of the GPU
+linear engine's move set (tests/test_lin_balance.py), so with at most two
+monomers per site it samples the GPU linear engine's equilibrium.
+
+    python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear|ring2|lin2] [workdir]
 """
 import glob
 import json
@@ -58,6 +66,7 @@ def stacking(files):
 
 
 T2, N_THERM2, NSEED2 = 200000, 101, 32
+NSEEDL2 = 16         # lin2: linear chains, 400000 time units each
 
 
 def frames_of(d, t_units, interval):
@@ -78,39 +87,45 @@ def seed_stats(ring, files, n_therm):
             "g1": g1[LAGS].tolist(), "g3": g3[LAGS].tolist(), "stack3": stacking(files[n_therm:])}
 
 
-def main_ring2(work):
-    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring")
+def main_two_per_site(work, ring):
+    """ring2: sl_equal from RingHost starts; lin2: sl_double from LinHost starts (see the module text)."""
+    exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring" if ring else "efpol_port_lin")
+    nseed, t_units, interval, n_therm = (NSEED2, T2, INTERVAL, N_THERM2) if ring else (NSEEDL2, 10 * T, 10 * INTERVAL,
+                                                                                      N_THERM)
+    model = "sl_equal" if ring else "sl_double"
+    host = elp.RingHost if ring else elp.LinHost
     dirs = []
-    for k in range(NSEED2):
+    for k in range(nseed):
         d = os.path.join(work, f"k{k}")
         os.makedirs(d, exist_ok=True)
-        h = elp.RingHost(N=N, L=L, density=1.2, seed=k)
+        h = host(N=N, L=L, density=1.2, seed=k)
         h.write_tuv(os.path.join(d, "start.res"))
         h.close()
         dirs.append(d)
-    for b in range(0, NSEED2, 8):                 # 8 processes at a time
-        procs = [subprocess.Popen([exe, "from", os.path.join(dirs[k], "start.res"), str(2 * k + 2), dirs[k], str(T2),
-                                   str(INTERVAL), "2"], stdout=subprocess.DEVNULL) for k in range(b, min(b + 8, NSEED2))]
+    for b in range(0, nseed, 8):                  # 8 processes at a time
+        procs = [subprocess.Popen([exe, "from", os.path.join(dirs[k], "start.res"), str(2 * k + 2), dirs[k],
+                                   str(t_units), str(interval), "2", model], stdout=subprocess.DEVNULL)
+                 for k in range(b, min(b + 8, nseed))]
         for p in procs:
             assert p.wait() == 0
-    out = {"L": L, "N": N, "density": 1.2, "model": "sl_equal, at most two monomers per site", "time_units": T2,
-           "interval": INTERVAL, "n_therm_frames": N_THERM2, "lags": LAGS,
-           "start": f"RingHost(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2, k = 0..{NSEED2 - 1}",
+    out = {"L": L, "N": N, "density": 1.2, "model": f"{model}, at most two monomers per site", "time_units": t_units,
+           "interval": interval, "n_therm_frames": n_therm, "lags": LAGS,
+           "start": f"{host.__name__}(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2, k = 0..{nseed - 1}",
            "seeds": []}
     for k, d in enumerate(dirs):
-        row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(True, frames_of(d, T2, INTERVAL), N_THERM2)}
+        row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(ring, frames_of(d, t_units, interval), n_therm)}
         assert row["stack3"] == 0
         out["seeds"].append(row)
         print(k, row["rg2"], row["slrat"], flush=True)
-    with open(os.path.join(HERE, "ring2_L32_N100.json"), "w") as f:
+    with open(os.path.join(HERE, ("ring2" if ring else "lin2") + "_L32_N100.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
 def main():
     args = sys.argv[1:]
-    kind = args.pop(0) if args and args[0] in ("ring", "linear", "ring2") else "ring"
-    if kind == "ring2":
-        return main_ring2(args[0] if args else tempfile.mkdtemp())
+    kind = args.pop(0) if args and args[0] in ("ring", "linear", "ring2", "lin2") else "ring"
+    if kind in ("ring2", "lin2"):
+        return main_two_per_site(args[0] if args else tempfile.mkdtemp(), kind == "ring2")
     ring = kind == "ring"
     work = args[0] if args else tempfile.mkdtemp()
     t_units, interval = (T, INTERVAL) if ring else (10 * T, 10 * INTERVAL)
