@@ -38,6 +38,7 @@ PKG = os.path.join(ROOT, "mc-polymer-simulation_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense bf16 MFMA
 
 
 def cpu_model():
@@ -254,6 +255,7 @@ def bench_learned(args):
     t_train = time.perf_counter() - t0
     e96.close()
     curves = {}
+    lp_kernel_ms, lp_launches, lp_blocks = 0.0, 0, 0
     for kind in ("plain", "learned"):
         h = elp.RingHost(N=4000, L=128, density=1.2, seed=0)
         sub, n_iter = (elp.SUBLAT_DENSE, 13) if kind == "plain" else (elp.SUBLAT_CUBIC, 51)
@@ -270,6 +272,9 @@ def bench_learned(args):
                 eng.run(org, tau0=tau, seed=h.seed)
             else:
                 net.run(eng, org, tau0=tau, seed=h.seed)
+                lp_kernel_ms += eng.last_run_ms()        # HIP events around the launches of this chunk
+                lp_launches += chunk
+                lp_blocks = eng.launch_info()["blocks"]
             gpu_s += time.perf_counter() - t1
             tau += chunk
             pts.append((gpu_s, tau, _rg2_now(h, eng)))
@@ -286,6 +291,12 @@ def bench_learned(args):
         return pts[-1][0], pts[-1][1]
 
     (tp, up), (tl, ul) = t_reach(curves["plain"]), t_reach(curves["learned"])
+    # MFMA roofline of the learned kernel: per proposal two network evaluations of
+    # 144*64 + 64*64 + 64*11 multiply-adds; one launch = every 16^3 tile x 64 slots
+    # x 51 iterations x 4 substeps
+    launch_s = lp_kernel_ms * 1e-3 / max(lp_launches, 1)
+    flops_per_launch = 2 * 2 * (144 * 64 + 64 * 64 + 64 * 11) * lp_blocks * 64 * 51 * 4
+    achieved_tf = flops_per_launch / launch_s / 1e12
     rec = {
         "metric": "MC move attempts/s, %HBM roofline; equil. wall-time learned vs plain",
         "value": tp / tl,
@@ -307,11 +318,66 @@ def bench_learned(args):
             "curves": {k: [[round(a, 3), b, round(c, 4)] for a, b, c in v] for k, v in curves.items()},
             "parallelism": "single GPU",
         },
-        "roofline": None,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved_tf,
+            "peak": BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / BF16_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "lp_ring_time_unit2",
+            "flops_per_launch": flops_per_launch,
+            "avg_launch_us": launch_s * 1e6,
+            "launches": lp_launches,
+        },
         "cpu_baseline": None,
     }
     print(json.dumps(rec))
     net.close()
+    return 0
+
+
+def bench_learned_units(args):
+    """Throughput of the learned-proposal kernel alone: ring melt L=128,
+    N=4000, density 1.2 from the compact start, 16^3 tiles, cubic sub-lattice,
+    51 iterations, the network's own initial parameters; one time unit per
+    step.  The MH chain's cost does not depend on the weights' values."""
+    import elp
+    tile = (16, 16, 16)
+    h = elp.RingHost(N=4000, L=128, density=1.2, seed=0)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_CUBIC, 51)
+    eng.set_rules(elp.RULES_BALANCED)
+    eng.upload(h.lattice())
+    h.load_shift(tile)
+    net = elp.Net(seed=1)
+    org = h.origins(tile, args.warmup + args.steps)
+    if args.warmup:
+        net.run(eng, org[:args.warmup], tau0=0, seed=h.seed)
+    eng.sync()
+    t0 = time.perf_counter()
+    prop, acc = net.run(eng, org[args.warmup:], tau0=args.warmup, seed=h.seed)
+    wall = time.perf_counter() - t0
+    launch_s = eng.last_run_ms() * 1e-3 / args.steps
+    blocks = eng.launch_info()["blocks"]
+    flops_per_launch = 2 * 2 * (144 * 64 + 64 * 64 + 64 * 11) * blocks * 64 * 51 * 4
+    achieved_tf = flops_per_launch / launch_s / 1e12
+    rec = {
+        "metric": "learned proposals/s (MH chain of the learned-proposal engine)",
+        "value": prop / wall, "unit": "proposals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8 lattice, bf16/fp32 network",
+        "data": "synthetic",
+        "config": {"workload": "ring melt L=128 N=4000 density 1.2 (629 rings), learned-proposal time units, "
+                               "16^3 tiles, cubic sub-lattice, 51 iterations, untrained network",
+                   "accepted_fraction": acc / prop if prop else None, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "lp_ring_time_unit2",
+                     "flops_per_launch": flops_per_launch, "avg_launch_us": launch_s * 1e6},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec))
+    net.close()
+    eng.close()
     return 0
 
 
@@ -436,10 +502,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-only", action="store_true",
                     help="only run the plain copy kernel (byte-counter calibration)")
-    ap.add_argument("--workload", default="melt", choices=["melt", "batch", "learned"],
+    ap.add_argument("--workload", default="melt", choices=["melt", "batch", "learned", "learned-units"],
                     help="melt: one box (default); batch: 64 ring replicas, L in {32,48,64,96}, "
                          "N in {50,200,1000,2000}, seeds 0..63, in one launch per time unit; learned: "
-                         "equilibration A/B of the learned proposal against the plain engine")
+                         "equilibration A/B of the learned proposal against the plain engine; learned-units: "
+                         "time units of the learned-proposal kernel alone (ring L=128 N=4000, untrained network)")
     ap.add_argument("--budget", type=float, default=30.0, help="learned: GPU seconds per arm")
     ap.add_argument("--units-per-graph", type=int, default=16,
                     help="batch: time units captured per HIP graph (0 = direct launches)")
@@ -465,6 +532,8 @@ def main():
     tile = tuple(int(x) for x in args.tile.split("x")) if "x" in args.tile else (int(args.tile),) * 3
     if args.workload == "batch":
         return bench_batch(args, tile)
+    if args.workload == "learned-units":
+        return bench_learned_units(args)
     if args.workload == "learned":
         return bench_learned(args)
     ring = args.chain == "ring"

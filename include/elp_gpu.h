@@ -329,6 +329,12 @@ int elp_run_learned(elp_engine* e, elp_net* n, int n_time, const int32_t* origin
 int elp_net_patch_moves(int device, const uint8_t* patches, const int32_t* moves, int count, uint8_t* out,
                         int32_t* ok, int32_t* rev);
 int elp_net_patch_features(int device, const uint8_t* patches, int count, int8_t* x);
+/* Test entry point: checks the learned engine's tile path (features from the
+   occupancy bitmap, moves applied to a 16^3 tile, bitmap flips, revert)
+   against the patch functions above; cases = count x (tile, t, u, v, move),
+   result = count x (failed-check bits, move possible). */
+int elp_net_tile_check(int device, const uint8_t* tiles, int n_tiles, const int32_t* cases, int count,
+                       int32_t* result);
 
 /* Diagnostic (no reference counterpart): device-to-device copy of nbytes
    (multiple of 16) with a plain streaming kernel, reps times; *ms = average

@@ -1316,6 +1316,28 @@ int elp_net_patch_features(int device, const uint8_t* patches, int count, int8_t
     return ELP_OK;
 }
 
+int elp_net_tile_check(int device, const uint8_t* tiles, int n_tiles, const int32_t* cases, int count,
+                       int32_t* result) {
+    if (!tiles || !cases || !result || n_tiles <= 0 || count <= 0) return fail(ELP_E_ARG, "bad tile-check arguments");
+    for (int i = 0; i < count; i++) {
+        const int32_t* c = cases + 5 * i;
+        if (c[0] < 0 || c[0] >= n_tiles || c[1] < 0 || c[1] >= 16 || c[2] < 0 || c[2] >= 16 || c[3] < 0 || c[3] >= 16 ||
+            c[4] < 0 || c[4] >= elp::LP_NMOVE)
+            return fail(ELP_E_ARG, "tile-check case %d out of range", i);
+    }
+    HIPCHK(hipSetDevice(device));
+    DevBuf dt, dc, dr;
+    HIPCHK(hipMalloc(&dt.p, (size_t)n_tiles * 4096));
+    HIPCHK(hipMalloc(&dc.p, sizeof(int32_t) * 5 * (size_t)count));
+    HIPCHK(hipMalloc(&dr.p, sizeof(int32_t) * 2 * (size_t)count));
+    HIPCHK(hipMemcpy(dt.p, tiles, (size_t)n_tiles * 4096, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dc.p, cases, sizeof(int32_t) * 5 * (size_t)count, hipMemcpyHostToDevice));
+    int rc = elp::lp_launch_tile_check((const uint8_t*)dt.p, (const int32_t*)dc.p, count, (int32_t*)dr.p, 0);
+    if (rc) return fail(ELP_E_HIP, "tile-check kernel: %s", hipGetErrorString((hipError_t)rc));
+    HIPCHK(hipMemcpy(result, dr.p, sizeof(int32_t) * 2 * (size_t)count, hipMemcpyDeviceToHost));
+    return ELP_OK;
+}
+
 __global__ void elp_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];

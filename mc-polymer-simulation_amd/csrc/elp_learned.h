@@ -30,6 +30,7 @@ int lp_launch_harvest(const uint8_t* lat, int LT, int LU, int LV, int n, uint32_
 int lp_launch_patch_move(const uint8_t* in, const int32_t* mv, int n, uint8_t* out, int32_t* ok, int32_t* rev,
                          hipStream_t st);
 int lp_launch_patch_features(const uint8_t* in, int n, int8_t* x, hipStream_t st);
+int lp_launch_tile_check(const uint8_t* tiles, const int32_t* cases, int n, int32_t* res, hipStream_t st);
 int lp_launch_mc(uint8_t* glat, const RingLaunch& p, int nblocks, const float* prm, unsigned long long* stats,
                  hipStream_t st);
 

@@ -30,6 +30,7 @@
 // are independent.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include "elp_device.h"
 #include "elp_learned.h"
 
@@ -733,6 +734,526 @@ __global__ __launch_bounds__(256) void lp_ring_time_unit(uint8_t* __restrict__ g
     }
 }
 
+// ------------------------------------------- learned MC, wave-owned slots
+// lp_ring_time_unit2: the chain of lp_ring_time_unit (same moves, proposal,
+// acceptance rule and random streams) with every activation in registers.
+// Wave w owns slots 16w..16w+15 of every substep.  Each layer is computed
+// transposed, D = W^T X^T on v_mfma_f32_16x16x32_bf16 (gfx950): the weights
+// are the A operand (rows = output units), the activations of the wave's 16
+// slots the B operand (columns = slots).  Lane l = 16g + c holds
+//   A(unit c of a 16-unit block, k-slot (g, kk, j)),  B(k-slot (g, kk, j), slot c),  j < 8,
+//   D(unit 4g + i of the block, slot c),  i < 4,
+// and the product sums over all k-slots, so which input unit a k-slot stands
+// for is free as long as A and B agree.  Layer 1 gives k-slot (g, kk, j)
+// feature j of patch row 4kk + g: rows of the five patch sites along t,
+// padded to eight, then rows 25-27 with the bond one-hot, the two
+// stored-length bits and the constant; a lane takes its row from one word of
+// an occupancy bitmap of the tile and turns the bits into bf16 0/1 values
+// with one read of a 256-entry LDS table.  Layers 2 and 3 give it hidden unit
+// 16(2kk + j/4) + 4g + j%4, exactly the units the lane holds in D of the
+// layer before, so a layer's output is the next layer's B operand without
+// leaving the lane.  The weights are permuted to these maps at the start of
+// a launch: layer 1's fragments stay in registers, those of layers 2 and 3
+// in LDS (one 16-byte read per lane and product).
+// The logits of a slot sit in its four lanes c + 16g (moves 4g..4g+3);
+// softmax and sampling combine them with lane shuffles.  Lane c (g = 0)
+// applies the proposed move to the tile at once and restores the old bytes
+// when the MH test rejects it, so the second pass reads S' from the tile.
+// A slot's writes (radius 1) never meet another slot's patch (radius 2,
+// sites at pitch 4) and bitmap bits flip by LDS atomics, so the slots of a
+// substep stay independent; DS operations of one wave complete in issue
+// order, so hand-offs between its lanes need only a compiler barrier; one
+// workgroup barrier per substep orders the tile writes of a substep before
+// the patch reads of the next.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int LP_K1 = 7;           // layer-1 k-steps: 28 rows of 8 feature slots
+constexpr int LP_OCCW = 20;        // bitmap rows per axis: u, v in [-2, 18)
+
+struct LpMc2Shared {
+    uint4 a2[4][2][64];            // layer-2 A fragments [unit block][k-step][lane]
+    uint4 a3[2][64];               // layer-3 A fragments [k-step][lane]
+    float b2[LP_NH], b3[LP_NOUT];
+    // occupancy of tile row (u, v) at index (v + 2) * 20 + (u + 2): bit 8 + t
+    // for t = 0..15, every other bit (and every row outside the tile) set:
+    // sites outside the tile count as occupied, as in the patch features
+    uint32_t occ[LP_OCCW * LP_OCCW];
+    uint4 x8[256];                 // eight 0/1 features of a byte as a B fragment (lp_expand8)
+    uint8_t tile[4096];
+};
+
+// input feature of layer-1 k-slot (g, kk, j), or -1 for a zero slot
+__device__ __forceinline__ int lp_feat1(int g, int kk, int j) {
+    const int r = 4 * kk + g;
+    if (r < 25) return j < 5 ? 5 * r + j : -1;     // patch site t = j of row r = u + 5v
+    if (r < 27) return 125 + 8 * (r - 25) + j;     // bond one-hot
+    return j < 3 ? 141 + j : -1;                   // stored length, partner's, constant
+}
+// hidden unit of layer-2/3 k-slot (g, kk, j)
+__device__ __forceinline__ int lp_hid(int g, int kk, int j) { return 16 * (2 * kk + (j >> 2)) + 4 * g + (j & 3); }
+
+// Bitmap word i ((v + 2) * 20 + (u + 2)) of a 16^3 tile.
+__device__ __forceinline__ uint32_t lp_occ_word(const uint8_t* tile, int i) {
+    const int v = i / LP_OCCW - 2, u = i % LP_OCCW - 2;
+    if ((unsigned)u >= 16u || (unsigned)v >= 16u) return 0xFFFFFFFFu;
+    uint32_t o = 0;
+    for (int t = 0; t < 16; t++) o |= (tile[t + 16 * (u + 16 * v)] != 0 ? 1u : 0u) << t;
+    return 0xFF0000FFu | (o << 8);
+}
+
+// Bitmap offset of patch row 4kk + g from the patch's row (u, v) = (-2, -2)
+// (row 24 where 4kk + g > 24: those lanes replace it by their tail row).
+__device__ __forceinline__ int lp_row_offset(int g, int kk) {
+    const int r = (4 * kk + g < 25) ? 4 * kk + g : 24;
+    return LP_OCCW * (r / 5) + r % 5;
+}
+
+// Features of layer-1 k-step kk in lane group g as eight bits: the patch row
+// from the bitmap (occ0: index of the slot's row (u, v) = (-2, -2), tsh: bit
+// of t = -2), or the tail row (g >= 1 at the last k-step).
+__device__ __forceinline__ uint32_t lp_row_bits(const uint32_t* occ, int occ0, int rowoff, int tsh, uint32_t tail,
+                                                int g, int kk) {
+    const uint32_t bits = (occ[occ0 + rowoff] >> tsh) & 0x1fu;
+    return (kk == LP_K1 - 1 && g != 0) ? tail : bits;
+}
+
+__device__ __forceinline__ uint32_t lp_pack2(float lo, float hi) {
+    return (uint32_t)to_bf16(lo) | ((uint32_t)to_bf16(hi) << 16);
+}
+
+// Eight 0/1 features from the low byte of b: bf16 1.0 (0x3F80) where a bit is set.
+__device__ __forceinline__ bf16x8 lp_expand8(uint32_t b) {
+    uint4 d;
+    d.x = (((b >> 0) & 1u) | (((b >> 1) & 1u) << 16)) * 0x3F80u;
+    d.y = (((b >> 2) & 1u) | (((b >> 3) & 1u) << 16)) * 0x3F80u;
+    d.z = (((b >> 4) & 1u) | (((b >> 5) & 1u) << 16)) * 0x3F80u;
+    d.w = (((b >> 6) & 1u) | (((b >> 7) & 1u) << 16)) * 0x3F80u;
+    return __builtin_bit_cast(bf16x8, d);
+}
+
+// relu of two D tiles (units 4g+i of blocks 2kk and 2kk+1) as a B operand
+__device__ __forceinline__ bf16x8 lp_relu_pack(const f32x4& x, const f32x4& y) {
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        r[i] = (__bf16)__builtin_amdgcn_fmed3f(x[i], 0.f, 3.0e38f);
+        r[4 + i] = (__bf16)__builtin_amdgcn_fmed3f(y[i], 0.f, 3.0e38f);
+    }
+    return r;
+}
+
+__device__ __forceinline__ bf16x8 lds_frag(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
+
+// max of two finite values in one instruction (median of a, b and +inf)
+__device__ __forceinline__ float lp_max(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, __builtin_inff()); }
+
+// LDS hand-off between lanes of one wave: its DS operations complete in
+// issue order, so only the compiler must keep them in program order.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Logits of moves 4g..4g+3 of this lane's slot.  occ0: bitmap index of the
+// slot's patch row (u, v) = (-2, -2), rowoff[kk]: offset of patch row 4kk + g
+// from it, tsh: bit of the patch's t = -2 in a bitmap word, tail: features of
+// row 25 + (g - 1) (lanes g >= 1 at kk = 6).
+__device__ __forceinline__ f32x4 lp_forward_t(const LpMc2Shared& M, const bf16x8 (&a1)[4][LP_K1],
+                                              const int (&rowoff)[LP_K1], int occ0, int tsh, uint32_t tail, int g,
+                                              int lane) {
+    f32x4 d1[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) d1[rb] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < LP_K1; kk++) {
+        const bf16x8 b = lds_frag(M.x8[lp_row_bits(M.occ, occ0, rowoff[kk], tsh, tail, g, kk)]);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) d1[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[rb][kk], b, d1[rb], 0, 0, 0);
+    }
+    bf16x8 h[2] = {lp_relu_pack(d1[0], d1[1]), lp_relu_pack(d1[2], d1[3])};
+    f32x4 d2[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        d2[rb] = *reinterpret_cast<const f32x4*>(&M.b2[16 * rb + 4 * g]);
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++)
+            d2[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lds_frag(M.a2[rb][kk][lane]), h[kk], d2[rb], 0, 0, 0);
+    }
+    h[0] = lp_relu_pack(d2[0], d2[1]);
+    h[1] = lp_relu_pack(d2[2], d2[3]);
+    f32x4 z = *reinterpret_cast<const f32x4*>(&M.b3[4 * g]);
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lds_frag(M.a3[kk][lane]), h[kk], z, 0, 0, 0);
+    return z;
+}
+
+// Tile byte at (t, u, v), WALL outside the tile.
+__device__ __forceinline__ uint32_t lp_tsite(const uint8_t* tile, int t, int u, int v) {
+    return ((unsigned)t < 16u && (unsigned)u < 16u && (unsigned)v < 16u) ? (uint32_t)tile[t + 16 * (u + 16 * v)] : WALL;
+}
+
+// Features of rows 25-27 of the slot at (t, u, v) (the lane's row for g >= 1).
+__device__ __forceinline__ uint32_t lp_tail_bits(const uint8_t* tile, int t, int u, int v, int g) {
+    const uint32_t s = tile[t + 16 * (u + 16 * v)];
+    const uint32_t next = s & 0xfu;
+    const uint32_t ps = lp_tsite(tile, t + bdt(next), u + bdu(next), v + bdv(next));
+    const uint32_t hot = s ? (1u << next) : 0u;
+    const uint32_t r27 = ((s >> 6) & 1u) | ((s && ps != WALL && (ps & 0x40u)) ? 2u : 0u) | 4u;
+    return g == 1 ? (hot & 0xffu) : g == 2 ? (hot >> 8) : r27;
+}
+
+// Move m of lp_move at tile site (t, u, v) of a 16^3 tile, in select form:
+// the three move types are evaluated side by side and one guarded block
+// writes the outcome to the tile.  Returns 1 if the move is possible, with
+// the reverse move in *rev, the sites it wrote in chi[0..*nch-1] (centre,
+// bond partner, third site) and their old bytes in cho; else 0 and the tile
+// is unchanged.  *flip: the one site whose occupancy the move changes (the
+// TransForw target fills, the TransBack source empties; -1 for DiffuseSL and
+// impossible moves).  The same moves as lp_move on the patch around the site.
+__device__ __forceinline__ int lp_move_tile(uint8_t* tile, int t, int u, int v, int m, int* rev, int* chi,
+                                            uint32_t* cho, int* nch, int* flip) {
+    const int pc = t + 16 * (u + 16 * v);
+    const uint32_t s = tile[pc];
+    const uint32_t next = s & 0xfu, label = s & 0x30u, sl = s & 0x40u;
+    const int nt = t + bdt(next), nu = u + bdu(next), nv = v + bdv(next);
+    const bool nIn = ((unsigned)nt < 16u) & ((unsigned)nu < 16u) & ((unsigned)nv < 16u);
+    const int in = nIn ? nt + 16 * (nu + 16 * nv) : pc;
+    const uint32_t ns = tile[in];                      // the partner (meaningful when nIn)
+    const uint32_t nsl = ns & 0x40u, srcNext = ns & 0xfu, srcLabel = ns & 0x30u;
+    const bool isTF = m < 8, isTB = (m == 8) | (m == 9), intoP = m == 8;
+    // TransForw: common neighbour k = m & 3 of the site and its partner
+    const uint32_t k = (uint32_t)m & 3u;
+    const uint32_t f = trans_first(next, k & 1u), fc = bond_add((~f) & 0xfu, next);
+    const uint32_t nb1 = (k & 2u) ? f : fc, nb2 = (k & 2u) ? fc : f;
+    // TransBack: fold the partner into the site; newNext = next + srcNext
+    const uint32_t newNext = bond_add(next, srcNext);
+    const bool sumOk = !(((next | srcNext) != 0xfu) && (next & srcNext)) & (newNext != 0u) & (newNext != 0x3u) &
+                       (newNext != 0xcu);
+    // third site: the TransForw target or the TransBack fold site
+    const uint32_t xb = isTF ? nb1 : newNext;
+    const int xt = t + bdt(xb), xu = u + bdu(xb), xv = v + bdv(xb);
+    const bool xIn = ((unsigned)xt < 16u) & ((unsigned)xu < 16u) & ((unsigned)xv < 16u);
+    const int ix = xIn ? xt + 16 * (xu + 16 * xv) : pc;
+    const uint32_t x = tile[ix];
+    const bool base = (s != 0u) & nIn;
+    // TransForw outcome
+    const bool mf = (m < 4) & (sl != 0u);
+    const bool okF = isTF & base & ((sl | nsl) != 0u) & ((m < 4) | ((sl != 0u) & (nsl != 0u))) & xIn & (x == 0u);
+    const uint32_t pF = mf ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
+    const uint32_t dF = nb2 | (mf ? (label & 0x10u) : ((ns & 0x20u) >> 1));
+    const uint32_t nF = mf ? ns : (ns & 0x1fu);
+    // TransBack outcome
+    const bool okB = isTB & base & (nsl == 0u) & sumOk & xIn & (intoP ? (sl == 0u) : ((x & 0x40u) == 0u));
+    const uint32_t pB = intoP ? (newNext | (label << 1) | srcLabel | 0x40u) : (newNext | label | sl);
+    const uint32_t wB = intoP ? x : ((x & 0x3fu) | (srcLabel << 1) | 0x40u);
+    // reverse of a fold: TransForw through the vacated site, k = the position of
+    // next among the four common neighbours of the site and its new partner
+    const uint32_t g0 = trans_first(newNext, 0u), g1 = trans_first(newNext, 1u);
+    const uint32_t kb = (bond_add((~g0) & 0xfu, newNext) == next) ? 0u
+                        : (bond_add((~g1) & 0xfu, newNext) == next) ? 1u
+                        : (g0 == next) ? 2u : (g1 == next) ? 3u : 4u;
+    const int rB = (int)(intoP ? kb : (sl != 0u) ? kb + 4u : kb);
+    // DiffuseSL outcome
+    const bool okD = (m == 10) & base & ((nsl + sl) == 0x40u);
+    const uint32_t pD = sl ? (next | ((label >> 1) & 0x10u)) : (next | (label << 1) | ((srcLabel >> 1) & 0x10u) | 0x40u);
+    const uint32_t nD = sl ? (ns | ((label & 0x10u) << 1) | 0x40u) : (ns & 0x1fu);
+    const bool ok = okF | okB | okD;
+    chi[0] = pc; cho[0] = s;
+    chi[1] = in; cho[1] = ns;
+    chi[2] = ix; cho[2] = x;
+    *nch = okD ? 2 : 3;
+    *rev = okF ? (mf ? 8 : 9) : okB ? rB : 10;
+    *flip = okF ? ix : okB ? in : -1;
+    if (ok) {
+        tile[pc] = (uint8_t)(okF ? pF : okB ? pB : pD);
+        tile[in] = (uint8_t)(okF ? nF : okB ? 0u : nD);
+        if (!okD) tile[ix] = (uint8_t)(okF ? dF : wB);
+    }
+    return ok ? 1 : 0;
+}
+
+// Flip the bitmap bit of tile site i (none for i < 0).
+__device__ __forceinline__ void lp_occ_flip(uint32_t* occ, int i) {
+    if (i >= 0) atomicXor(&occ[((i >> 8) + 2) * LP_OCCW + ((i >> 4) & 15) + 2], 1u << (8 + (i & 15)));
+}
+
+__global__ __launch_bounds__(256, 2) void lp_ring_time_unit2(uint8_t* __restrict__ glat, RingLaunch p,
+                                                              const float* __restrict__ prm,
+                                                              unsigned long long* __restrict__ stats) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    LpMc2Shared& M = *reinterpret_cast<LpMc2Shared*>(lds_raw);
+    constexpr int TT = 16;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
+    const uint32_t tileId = xcd_tile(blockIdx.x, gridDim.x);
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TT, gv0 = p.ov + kv * TT;
+
+    // layer-1 A fragments: unit 16rb + c, k-slots (g, kk, j)
+    bf16x8 a1[4][LP_K1];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int kk = 0; kk < LP_K1; kk++) {
+            uint4 d;
+            uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int f0 = lp_feat1(g, kk, 2 * q), f1 = lp_feat1(g, kk, 2 * q + 1);
+                dw[q] = lp_pack2(f0 >= 0 ? prm[LP_OFF_W1 + f0 * LP_NH + 16 * rb + c] : 0.f,
+                                 f1 >= 0 ? prm[LP_OFF_W1 + f1 * LP_NH + 16 * rb + c] : 0.f);
+            }
+            a1[rb][kk] = __builtin_bit_cast(bf16x8, d);
+        }
+    // layers 2 and 3: fragments of lane l = 16gg + rr in LDS
+    for (int e = tid; e < 4 * 2 * 64; e += 256) {
+        const int rb = e >> 7, kk = (e >> 6) & 1, l = e & 63, gg = l >> 4, rr = l & 15;
+        uint4 d;
+        uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            dw[q] = lp_pack2(prm[LP_OFF_W2 + lp_hid(gg, kk, 2 * q) * LP_NH + 16 * rb + rr],
+                             prm[LP_OFF_W2 + lp_hid(gg, kk, 2 * q + 1) * LP_NH + 16 * rb + rr]);
+        M.a2[rb][kk][l] = d;
+    }
+    if (tid < 128) {
+        const int kk = tid >> 6, l = tid & 63, gg = l >> 4, rr = l & 15;
+        uint4 d;
+        uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            dw[q] = lp_pack2(prm[LP_OFF_W3 + lp_hid(gg, kk, 2 * q) * LP_NOUT + rr],
+                             prm[LP_OFF_W3 + lp_hid(gg, kk, 2 * q + 1) * LP_NOUT + rr]);
+        M.a3[kk][l] = d;
+    }
+    if (tid < LP_NH) M.b2[tid] = prm[LP_OFF_B2 + tid];
+    if (tid < LP_NOUT) M.b3[tid] = prm[LP_OFF_B3 + tid];
+    M.x8[tid] = __builtin_bit_cast(uint4, lp_expand8((uint32_t)tid));
+    for (int i = tid; i < 4096; i += 256) {
+        int gt = gt0 + (i & 15), gu = gu0 + ((i >> 4) & 15), gv = gv0 + (i >> 8);
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        M.tile[i] = glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)];
+    }
+    __syncthreads();
+    for (int i = tid; i < LP_OCCW * LP_OCCW; i += 256) M.occ[i] = lp_occ_word(M.tile, i);
+    int rowoff[LP_K1];
+#pragma unroll
+    for (int kk = 0; kk < LP_K1; kk++) rowoff[kk] = lp_row_offset(g, kk);
+    const int slot = 16 * w + c;
+    uint32_t nacc = 0, nprop = 0;
+    __syncthreads();
+    for (int it = 0; it < p.n_iter; it++) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it >> 2)), p.seed, p.stream);
+        const uint32_t offw = word_of(ow, it & 3);
+        // the slot's two Philox blocks of this iteration: counter 2it in lane c
+        // (g = 0, substeps 0-1), 2it + 1 in lane c + 16 (g = 1, substeps 2-3)
+        const uint4 rw = philox4x32_10(make_uint4(p.tau, tileId, (uint32_t)slot | 0x40000000u,
+                                                  2u * (uint32_t)it + (uint32_t)(g & 1)), p.seed, p.stream);
+#pragma unroll 1
+        for (int s = 0; s < 4; s++) {
+            const uint32_t c6 = (offw >> (8 * s)) & 0x3fu;
+            const int st = 4 * (slot & 3) + (int)(c6 & 3u), su = 4 * ((slot >> 2) & 3) + (int)((c6 >> 2) & 3u),
+                      sv = 4 * (slot >> 4) + (int)((c6 >> 4) & 3u);
+            const int occ0 = LP_OCCW * sv + su, tsh = st + 6;
+            const int src = c + ((s >> 1) << 4);
+            const uint32_t um = __shfl(word_of(rw, 2 * (s & 1)), src, 64);
+            const uint32_t ua = __shfl(word_of(rw, 2 * (s & 1) + 1), src, 64);
+            // q(. | S): softmax over the slot's four lanes (moves 4g + i < 11)
+            f32x4 z = lp_forward_t(M, a1, rowoff, occ0, tsh, lp_tail_bits(M.tile, st, su, sv, g), g, lane);
+            float q[4];
+            int m;
+            float pm;
+            {
+                float mx = -3.0e38f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) mx = (4 * g + i < LP_NMOVE) ? lp_max(mx, z[i]) : mx;
+                mx = lp_max(mx, __shfl_xor(mx, 16, 64));
+                mx = lp_max(mx, __shfl_xor(mx, 32, 64));
+                float e[4], sg = 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    e[i] = (4 * g + i < LP_NMOVE) ? __expf(z[i] - mx) : 0.f;
+                    sg += e[i];
+                }
+                const float S0 = __shfl(sg, c, 64), S1 = __shfl(sg, c + 16, 64), S2 = __shfl(sg, c + 32, 64),
+                            S3 = __shfl(sg, c + 48, 64);
+                const float inv = __builtin_amdgcn_rcpf(((S0 + S1) + S2) + S3);
+                float Q = 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    q[i] = e[i] * inv;
+                    Q += q[i];
+                }
+                // m ~ q: the first move whose interval of the running sum holds u
+                // (the last of nonzero probability if rounding leaves u past the end)
+                const float Q0 = __shfl(Q, c, 64), Q1 = __shfl(Q, c + 16, 64), Q2 = __shfl(Q, c + 32, 64),
+                            Q3 = __shfl(Q, c + 48, 64);
+                const float E1 = Q0, E2 = E1 + Q1, E3 = E2 + Q2, E4 = E3 + Q3;
+                const float u = (float)(um >> 8) * (1.f / 16777216.f) * E4;
+                const float Eg = g == 0 ? 0.f : g == 1 ? E1 : g == 2 ? E2 : E3;
+                const float Eg1 = g == 0 ? E1 : g == 1 ? E2 : g == 2 ? E3 : E4;
+                const int last = Q3 > 0.f ? 3 : Q2 > 0.f ? 2 : Q1 > 0.f ? 1 : 0;
+                const bool win = (u >= Eg && u < Eg1) || (u >= E4 && g == last);
+                float ur = u - Eg;
+                int mi = -1, lastnz = -1;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    lastnz = (q[i] > 0.f) ? i : lastnz;
+                    const bool open = mi < 0;
+                    mi = (open & (ur < q[i])) ? i : mi;
+                    ur = (mi < 0) ? ur - q[i] : ur;
+                }
+                mi = (mi < 0) ? lastnz : mi;
+                const float pl = mi == 0 ? q[0] : mi == 1 ? q[1] : mi == 2 ? q[2] : q[3];
+                int cand = win ? 4 * g + mi : -1;
+                cand = max(cand, __shfl_xor(cand, 16, 64));
+                m = max(cand, __shfl_xor(cand, 32, 64));
+                float pc = win ? pl : 0.f;
+                pc += __shfl_xor(pc, 16, 64);
+                pm = pc + __shfl_xor(pc, 32, 64);
+            }
+            // lane c applies m to the tile (S -> S')
+            int ok = 0, rv = -1, nch = 0, flip = -1, chi[3] = {0, 0, 0};
+            uint32_t cho[3] = {0, 0, 0};
+            if (g == 0) {
+                ok = lp_move_tile(M.tile, st, su, sv, m, &rv, chi, cho, &nch, &flip);
+                lp_occ_flip(M.occ, flip);
+            }
+            wave_lds_sync();
+            // q(rev | S'), accept with min(1, q(rev | S') / q(m | S)) or restore S
+            z = lp_forward_t(M, a1, rowoff, occ0, tsh, lp_tail_bits(M.tile, st, su, sv, g), g, lane);
+            {
+                float mx = -3.0e38f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) mx = (4 * g + i < LP_NMOVE) ? lp_max(mx, z[i]) : mx;
+                mx = lp_max(mx, __shfl_xor(mx, 16, 64));
+                mx = lp_max(mx, __shfl_xor(mx, 32, 64));
+                float e[4], sg = 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    e[i] = (4 * g + i < LP_NMOVE) ? __expf(z[i] - mx) : 0.f;
+                    sg += e[i];
+                }
+                const float S0 = __shfl(sg, c, 64), S1 = __shfl(sg, c + 16, 64), S2 = __shfl(sg, c + 32, 64),
+                            S3 = __shfl(sg, c + 48, 64);
+                const float inv = __builtin_amdgcn_rcpf(((S0 + S1) + S2) + S3);
+                const int rvb = __shfl(rv, c, 64);
+                float qc = 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) qc = (rvb == 4 * g + i) ? e[i] * inv : qc;
+                qc += __shfl_xor(qc, 16, 64);
+                const float qr = qc + __shfl_xor(qc, 32, 64);
+                if (g == 0) {
+                    nprop++;
+                    if (ok) {
+                        const float uf = (float)(ua >> 8) * (1.f / 16777216.f);
+                        if (uf * pm < qr) {
+                            nacc++;
+                        } else {
+                            // the sites are distinct, so the order of restoring is free
+                            lp_occ_flip(M.occ, flip);
+#pragma unroll
+                            for (int j = 0; j < 3; j++)
+                                if (j < nch) M.tile[chi[j]] = (uint8_t)cho[j];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < 4096; i += 256) {
+        int gt = gt0 + (i & 15), gu = gu0 + ((i >> 4) & 15), gv = gv0 + (i >> 8);
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        glat[(size_t)gt + (size_t)p.LT * ((size_t)gu + (size_t)p.LU * (size_t)gv)] = M.tile[i];
+    }
+    if (stats) {
+        // one sum per wave, added by its first lane
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            nprop += __shfl_xor(nprop, d, 64);
+            nacc += __shfl_xor(nacc, d, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&stats[0], (unsigned long long)nprop);
+            atomicAdd(&stats[1], (unsigned long long)nacc);
+        }
+    }
+}
+
+// Check of the learned engine's tile path against the patch functions (tests):
+// per case (tile, t, u, v, m), one workgroup compares the features built from
+// the bitmap with lp_features of the patch around the site, lp_move_tile with
+// lp_move on that patch (outcome, reverse move, written bytes; nothing outside
+// the patch changes), the bitmap after the flips with the bitmap of the new
+// tile, and the tile and bitmap after the revert with the originals.
+// res[2i] = failed checks (bits 0-6 in that order), res[2i+1] = outcome.
+__global__ __launch_bounds__(64) void lp_tile_check_kernel(const uint8_t* __restrict__ tiles,
+                                                            const int32_t* __restrict__ cases, int32_t* __restrict__ res) {
+    __shared__ uint8_t tile[4096], orig[4096];
+    __shared__ uint32_t occ[LP_OCCW * LP_OCCW];
+    __shared__ uint8_t q[128];
+    __shared__ uint16_t x[LP_NIN];
+    const int ci = blockIdx.x, tid = threadIdx.x;
+    const int32_t* cs = cases + 5 * ci;
+    const uint8_t* src = tiles + (size_t)cs[0] * 4096;
+    for (int i = tid; i < 4096; i += 64) tile[i] = orig[i] = src[i];
+    __syncthreads();
+    for (int i = tid; i < LP_OCCW * LP_OCCW; i += 64) occ[i] = lp_occ_word(tile, i);
+    __syncthreads();
+    if (tid != 0) return;
+    const int t = cs[1], u = cs[2], v = cs[3], m = cs[4];
+    for (int i = 0; i < 125; i++) q[i] = (uint8_t)lp_tsite(tile, t - 2 + i % 5, u - 2 + i / 5 % 5, v - 2 + i / 25);
+    lp_features(q, x);
+    int flags = 0;
+    const int occ0 = LP_OCCW * v + u, tsh = t + 6;
+    for (int g = 0; g < 4; g++) {
+        const uint32_t tail = lp_tail_bits(tile, t, u, v, g);
+        for (int kk = 0; kk < LP_K1; kk++) {
+            const uint32_t bits = lp_row_bits(occ, occ0, lp_row_offset(g, kk), tsh, tail, g, kk);
+            for (int j = 0; j < 8; j++) {
+                const int f = lp_feat1(g, kk, j);
+                const uint32_t want = f >= 0 ? (x[f] != 0 ? 1u : 0u) : 0u;
+                if (((bits >> j) & 1u) != want) flags |= 1;
+            }
+        }
+    }
+    int rvp = -1, chp[3];
+    const int okp = lp_move(q, m, &rvp, chp);
+    int rvt = -1, chi[3], nch = 0, flip = -1;
+    uint32_t cho[3];
+    const int okt = lp_move_tile(tile, t, u, v, m, &rvt, chi, cho, &nch, &flip);
+    if (okp != okt) flags |= 2;
+    if (okp && okt && rvp != rvt) flags |= 4;
+    for (int i = 0; i < 125; i++) {
+        const uint32_t w = lp_tsite(tile, t - 2 + i % 5, u - 2 + i / 5 % 5, v - 2 + i / 25);
+        if (w != q[i]) flags |= 8;
+    }
+    for (int i = 0; i < 4096; i++) {
+        const int dt = (i & 15) - t, du = ((i >> 4) & 15) - u, dv = (i >> 8) - v;
+        const bool inPatch = dt >= -2 && dt <= 2 && du >= -2 && du <= 2 && dv >= -2 && dv <= 2;
+        if (!inPatch && tile[i] != orig[i]) flags |= 16;
+    }
+    lp_occ_flip(occ, flip);
+    for (int i = 0; i < LP_OCCW * LP_OCCW; i++)
+        if (occ[i] != lp_occ_word(tile, i)) flags |= 32;
+    if (okt) {
+        lp_occ_flip(occ, flip);
+        for (int j = 0; j < nch; j++) tile[chi[j]] = (uint8_t)cho[j];
+    }
+    for (int i = 0; i < 4096; i++)
+        if (tile[i] != orig[i]) flags |= 64;
+    for (int i = 0; i < LP_OCCW * LP_OCCW; i++)
+        if (occ[i] != lp_occ_word(orig, i)) flags |= 64;
+    res[2 * ci] = flags;
+    res[2 * ci + 1] = okt;
+}
+
 // ---------------------------------------------------------------- launchers
 template <class K>
 static int set_lds(K kernel, size_t lds) {
@@ -784,6 +1305,11 @@ int lp_launch_patch_move(const uint8_t* in, const int32_t* mv, int n, uint8_t* o
     return (int)hipGetLastError();
 }
 
+int lp_launch_tile_check(const uint8_t* tiles, const int32_t* cases, int n, int32_t* res, hipStream_t st) {
+    hipLaunchKernelGGL(lp_tile_check_kernel, dim3(n), dim3(64), 0, st, tiles, cases, res);
+    return (int)hipGetLastError();
+}
+
 int lp_launch_patch_features(const uint8_t* in, int n, int8_t* x, hipStream_t st) {
     hipLaunchKernelGGL(lp_patch_features_kernel, dim3((n + 63) / 64), dim3(64), 0, st, in, n, x);
     return (int)hipGetLastError();
@@ -791,10 +1317,22 @@ int lp_launch_patch_features(const uint8_t* in, int n, int8_t* x, hipStream_t st
 
 int lp_launch_mc(uint8_t* glat, const RingLaunch& p, int nblocks, const float* prm, unsigned long long* stats,
                  hipStream_t st) {
-    const size_t lds = sizeof(LpMcShared);
-    int rc = set_lds(lp_ring_time_unit, lds);
+    // default: wave-owned slots (ELP_LP_VARIANT=1: the workgroup-wide first version)
+    static const int variant = [] {
+        const char* env = getenv("ELP_LP_VARIANT");
+        return env ? atoi(env) : 2;
+    }();
+    if (variant == 1) {
+        const size_t lds = sizeof(LpMcShared);
+        int rc = set_lds(lp_ring_time_unit, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(lp_ring_time_unit, dim3(nblocks), dim3(256), lds, st, glat, p, prm, stats);
+        return (int)hipGetLastError();
+    }
+    const size_t lds = sizeof(LpMc2Shared);
+    int rc = set_lds(lp_ring_time_unit2, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(lp_ring_time_unit, dim3(nblocks), dim3(256), lds, st, glat, p, prm, stats);
+    hipLaunchKernelGGL(lp_ring_time_unit2, dim3(nblocks), dim3(256), lds, st, glat, p, prm, stats);
     return (int)hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ from .core import (  # noqa: F401
     Net,
     patch_moves,
     patch_features,
+    tile_check,
     Batch,
     Engine,
     RingHost,

@@ -127,6 +127,7 @@ def load_elp():
         lib.elp_run_learned.argtypes = [P, P, I, P, U64, U32, U32, C.POINTER(U64), C.POINTER(U64)]
         lib.elp_net_patch_moves.argtypes = [I, P, P, I, P, P, P]
         lib.elp_net_patch_features.argtypes = [I, P, I, P]
+        lib.elp_net_tile_check.argtypes = [I, P, I, P, I, P]
         lib.elp_autotune.argtypes = [I, I, I, P, P, I, I, I, C.POINTER(Schedule), C.c_char_p, U64]
         lib.elp_count_moves.argtypes = [P, I]
         lib.elp_move_counts.argtypes = [P, P]
@@ -484,6 +485,18 @@ def patch_features(patches, device=0):
     x = np.empty((len(p), NET_NIN), np.int8)
     _check(lib.elp_net_patch_features(device, p.ctypes.data, len(p), x.ctypes.data), lib)
     return x
+
+
+def tile_check(tiles, cases, device=0):
+    """Learned engine's tile path against the patch functions: per case
+    (tile, t, u, v, move) the failed-check bits and whether the move is
+    possible (elp_net_tile_check)."""
+    lib = load_elp()
+    t = np.ascontiguousarray(tiles, dtype=np.uint8).reshape(-1, 4096)
+    c = np.ascontiguousarray(cases, dtype=np.int32).reshape(-1, 5)
+    r = np.empty((len(c), 2), np.int32)
+    _check(lib.elp_net_tile_check(device, t.ctypes.data, len(t), c.ctypes.data, len(c), r.ctypes.data), lib)
+    return r[:, 0], r[:, 1]
 
 
 class _Host:

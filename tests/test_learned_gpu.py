@@ -125,6 +125,44 @@ def test_site_moves_match_oracle_and_reverse():
     assert np.array_equal(rev2, moves[sel])
 
 
+def test_tile_path_matches_patch_functions():
+    """The learned engine's kernel works on the tile itself: features from an
+    occupancy bitmap, moves written to the tile at once and restored on
+    rejection.  For 16^3 windows of a melt, sites anywhere in the tile (faces
+    included, where patch sites fall outside it) and every move, the kernel's
+    functions must equal the patch functions checked above: features, move
+    outcome, reverse move, written bytes, bitmap after the flips, and the
+    tile and bitmap after the revert."""
+    _say("[learned] tile path vs patch functions")
+    h = elp.RingHost(N=100, L=32, density=1.2, seed=6)
+    lat0 = h.lattice()
+    h.load_shift((16, 16, 16))
+    lat = replay_ring(lat0, h.L, (16, 16, 16), elp.SUBLAT_DENSE, 13, h.origins((16, 16, 16), 20), 0, h.seed,
+                      rules=elp.RULES_BALANCED).reshape(32, 32, 32)     # [v][u][t]
+    rng = np.random.default_rng(8)
+    tiles = []
+    for _ in range(48):
+        o = rng.integers(0, 32, 3)
+        w = np.roll(lat, (-o[2], -o[1], -o[0]), axis=(0, 1, 2))[:16, :16, :16]
+        tiles.append(np.ascontiguousarray(w).reshape(-1))
+    tiles = np.stack(tiles)
+    cases = []
+    for k in range(1400):
+        ti = int(rng.integers(0, len(tiles)))
+        occ = np.flatnonzero(tiles[ti])
+        site = int(rng.choice(occ)) if k % 10 else int(rng.integers(0, 4096))
+        t, u, v = site & 15, (site >> 4) & 15, site >> 8
+        for m in range(11):
+            cases.append((ti, t, u, v, m))
+    cases = np.array(cases, np.int32)
+    flags, ok = elp.tile_check(tiles, cases)
+    bad = np.flatnonzero(flags)
+    assert len(bad) == 0, (len(bad), cases[bad[:5]].tolist(), flags[bad[:5]].tolist())
+    face = np.any((cases[:, 1:4] == 0) | (cases[:, 1:4] == 15), axis=1)
+    _say(f"  {len(cases)} cases, {int(ok.sum())} possible moves, {int(ok[face].sum())} of them at a tile face")
+    assert ok.sum() > 800 and ok[face].sum() > 150
+
+
 def test_features_match_definition():
     P = _melt_patches(300, seed=5)
     P[:20, :25] = 0xFF                                  # outside-the-tile rows of a patch
