@@ -543,6 +543,16 @@ __global__ __launch_bounds__(THREADS) void ring_time_unit2(uint8_t* __restrict__
                                                                          xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// ring_time_unit2 with the occupied slots of every substep compacted across
+// the workgroup's waves (variant 5; measured against the default).
+template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__global__ __launch_bounds__(THREADS) void ring_time_unit_compact(uint8_t* __restrict__ glat, RingLaunch p) {
+    __shared__ __align__(16) uint8_t tile[PTile<TT, TU, TV, PAD>::NSITE + RingMovesT<true>::EXTRA];
+    __shared__ uint32_t list[THREADS + THREADS / 64];
+    tile_unit_compact<RingMovesT<true>, TT, TU, TV, DENSE, THREADS, PAD>(glat, tile, list, unit_args(p),
+                                                                         xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // One time unit of every replica of a batch (one workgroup per tile).
 template <int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool PAIRS = false, bool COUNT = false>
 __global__ __launch_bounds__(THREADS) void ring_batch_unit(uint8_t* __restrict__ gbase, BatchLaunch b) {
@@ -560,6 +570,14 @@ static void launch2(int variant, int nblocks, hipStream_t stream, uint8_t* glat,
         hipLaunchKernelGGL((ring_time_unit2<TT, TU, TV, DENSE, TH, PADC, true, true>), dim3(nblocks), dim3(TH), 0,
                            stream, glat, p);
         return;
+    }
+    constexpr bool ONE_SLOT = (TT / 4) * (TU / 4) * (TV / 4) * (DENSE ? 4 : 1) == TH;
+    if constexpr (ONE_SLOT) {
+        if (variant == 5) {
+            hipLaunchKernelGGL((ring_time_unit_compact<TT, TU, TV, DENSE, TH, PADC>), dim3(nblocks), dim3(TH), 0,
+                               stream, glat, p);
+            return;
+        }
     }
     if constexpr (TT == 16 && TU == 16 && TV <= 16) {
         if (variant == 4)
@@ -581,7 +599,9 @@ static void launch2(int variant, int nblocks, hipStream_t stream, uint8_t* glat,
 // ---- dispatch over the compiled schedules ----
 // variant 1: ring_time_unit (cell-major LDS); 2: ring_time_unit2 compact;
 // 3: ring_time_unit2 with P-indexed (padded) LDS where the tile allows it;
-// 4 (default): as 3 with the LDS bond-pair table of TransForw.
+// 4 (default): as 3 with the LDS bond-pair table of TransForw; 5: as 4 with
+// the occupied slots of each substep compacted across the waves (schedules
+// with one slot per thread; the others run variant 4).
 #define ELP_RING_CASE(TT, TU, TV, DENSE, TH)                                                              \
     if (tt == TT && tu == TU && tv == TV && dense == DENSE && threads == TH) {                             \
         if (probe) return 0;                                                                               \

@@ -210,6 +210,79 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     }
 }
 
+// tile_unit with the occupied slots of each substep compacted to the front of
+// the workgroup: each thread reads its slot's site, a ballot and a prefix sum
+// over the waves place the occupied slots (packed coordinate and random byte)
+// in list[0..count), and thread i < count makes the move of entry i.  An empty
+// site makes no move, so the results equal tile_unit's (random bits and
+// outcomes are keyed by the slot, not by the thread).  One slot per thread;
+// list holds THREADS + THREADS/64 words.  No counting instantiation.
+template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD>
+__device__ __forceinline__ void tile_unit_compact(typename M::site_t* __restrict__ glat, typename M::site_t* tile,
+                                                  uint32_t* list, const UnitArgs& p, uint32_t tileId) {
+    using G = PTile<TT, TU, TV, PAD>;
+    constexpr int NCT = TT / 4, NCU = TU / 4;
+    constexpr int NCELL = (TT / 4) * (TU / 4) * (TV / 4);
+    constexpr int SPC = DENSE ? 4 : 1;
+    constexpr int NW = THREADS / 64;
+    static_assert(NCELL * SPC == THREADS, "one slot per thread");
+    static_assert(THREADS % 64 == 0, "whole waves");
+    uint32_t* cnt = list + THREADS;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
+    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
+    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
+    tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
+    const uint32_t slot = (uint32_t)tid;
+    const uint32_t cell = slot / SPC, j = slot % SPC;
+    const uint32_t base = G::pack(4u * (cell % NCT) + (DENSE ? j : 0u), 4u * ((cell / NCT) % NCU), 4u * (cell / (NCT * NCU)));
+    const uint32_t ju = (3u * j) << G::SU, vb = base | ((2u * (j & 1u)) << G::SV);
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    M::template init<G>(tile, tid);
+    __syncthreads();
+    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const uint4 sw = philox4x32_10_lanes(make_uint4(p.tau, tileId, slot, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            if (jj >= nj) break;
+            const uint32_t offw = word_of(ow, jj);
+#pragma unroll
+            for (int s = 0; s < M::NSUB; s++) {
+                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+                const uint32_t cu = (c & 3u) << G::SU, cv = (c >> 2) << G::SV;
+                const uint32_t P = DENSE ? (((cu + ju) & (3u << G::SU)) | (vb ^ cv))
+                                         : base + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
+                const uint32_t r = (word_of(sw, jj) >> (8 * s)) & 0xffu;
+                const bool occ = tile[G::addr(P)] != 0;
+                const uint64_t bal = __ballot(occ);
+                if (lane == 0) cnt[w] = (uint32_t)__popcll(bal);
+                __syncthreads();
+                uint32_t before = (uint32_t)__popcll(bal & below), total = 0;
+#pragma unroll
+                for (int k = 0; k < NW; k++) {
+                    const uint32_t ck = cnt[k];
+                    before += (k < w) ? ck : 0u;
+                    total += ck;
+                }
+                if (occ) list[before] = P | (r << 24);
+                __syncthreads();
+                if ((uint32_t)tid < total) {
+                    const uint32_t e = list[tid];
+                    M::template move<G>(tile, e & 0xFFFFFFu, s, e >> 24, p.rules);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    int tidStore = tid;
+    asm volatile("" : "+v"(tidStore));
+    tile_copy<false, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tidStore);
+}
+
 // Workgroup of a batched launch: global tile g (XCD-aware order over all
 // replicas' tiles) -> replica r = rep_of_tile[g], its local tile g - tile0,
 // and the origin / time unit of replica r for time unit (*cursor + k).

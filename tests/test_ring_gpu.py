@@ -182,3 +182,49 @@ def test_ring_non_cubic_box_bit_exact():
     assert np.count_nonzero(want != lat0) > 1000
     assert np.array_equal(eng.download(), want)
     eng.close()
+
+
+_COMPACT_CHECK = r"""
+import sys
+import numpy as np
+import elp
+from ringcheck import replay_ring
+fails = 0
+for tile, sublat, n_iter, rules in (((16, 16, 16), elp.SUBLAT_DENSE, 13, elp.RULES_REFERENCE),
+                                    ((16, 16, 16), elp.SUBLAT_DENSE, 13, elp.RULES_BALANCED),
+                                    ((16, 16, 16), elp.SUBLAT_CUBIC, 51, elp.RULES_REFERENCE),
+                                    ((32, 32, 32), elp.SUBLAT_CUBIC, 51, elp.RULES_REFERENCE)):
+    h = elp.RingHost(N=100, L=64, density=1.2, seed=3)
+    lat0 = h.lattice()
+    h.load_shift(tile)
+    org = h.origins(tile, 6)
+    eng = elp.Engine("ring", h.L, tile, sublat, n_iter)
+    eng.set_rules(rules)
+    eng.upload(lat0)
+    eng.run(org, tau0=2, seed=h.seed)
+    got = eng.download()
+    want = replay_ring(lat0, h.L, tile, sublat, n_iter, org, 2, h.seed, rules=rules)
+    ok = np.array_equal(got, want) and np.count_nonzero(want != lat0) > 1000
+    print(f"compact {tile} sublattice {sublat} rules {rules}: {'bit-exact' if ok else 'DIFFERS'}", flush=True)
+    fails += not ok
+    eng.close()
+sys.exit(1 if fails else 0)
+"""
+
+
+def test_ring_compacted_variant_bit_exact():
+    """The variant that compacts the occupied slots of each substep across the
+    workgroup's waves (ELP_RING_VARIANT=5, a measured alternative to the
+    default) gives the replay's lattice on the schedules with one slot per
+    thread.  The variant is chosen once per process, so it runs in a child
+    process."""
+    import os
+    import subprocess
+    _say("[ring parity] compacted-slot variant, child process")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, ELP_RING_VARIANT="5",
+               PYTHONPATH=os.pathsep.join([os.path.join(root, "mc-polymer-simulation_amd"),
+                                           os.path.join(root, "tests")]))
+    r = subprocess.run([sys.executable, "-c", _COMPACT_CHECK], env=env, capture_output=True, text=True, timeout=300)
+    _say(r.stdout + r.stderr)
+    assert r.returncode == 0
