@@ -29,6 +29,11 @@ linear engine's move set (tests/test_lin_balance.py), so with at most two
 monomers per site it samples the GPU linear engine's equilibrium.
 
     python tests/golden/efpol_stats/make_efpol_stats.py [ring|linear|ring2|lin2] [workdir]
+    python tests/golden/efpol_stats/make_efpol_stats.py ring2 --N 50 [--T 200000] [--therm 101] [workdir]
+
+The chain-length series of ring2 (Rg^2(N)): N = 50 with the N = 100 protocol, and
+N = 200 with 300000 time units and the first 151 frames discarded (relaxation
+grows about as N^2), ring2_L32_N50.json and ring2_L32_N200.json.
 """
 import glob
 import json
@@ -69,36 +74,39 @@ T2, N_THERM2, NSEED2 = 200000, 101, 32
 NSEEDL2 = 16         # lin2: linear chains, 400000 time units each
 
 
-def frames_of(d, t_units, interval):
-    files = sorted(glob.glob(os.path.join(d, f"N_{N}", "t=*_dev=0.res")),
+def frames_of(d, t_units, interval, n=N):
+    files = sorted(glob.glob(os.path.join(d, f"N_{n}", "t=*_dev=0.res")),
                    key=lambda f: int(os.path.basename(f)[2:].split("_")[0]))
     assert len(files) == t_units // interval + 1
     return files
 
 
-def seed_stats(ring, files, n_therm):
+def seed_stats(ring, files, n_therm, n=N):
     npol = int(open(files[0]).read().split()[7])
-    ser = Series(ring, (L, L, L), npol, N, len(files))
+    ser = Series(ring, (L, L, L), npol, n, len(files))
     for f in files:
         ser.add_tuv(f)
     st = ser.statics(n_therm)
     g1, g3 = ser.msd(n_therm, max(LAGS) + 1)
-    return {"n_pol": npol, "rg2": st["rg2"], "slrat": st["slrat"], "pc": st["pc"][:N // 2 + 1].tolist(),
+    return {"n_pol": npol, "rg2": st["rg2"], "slrat": st["slrat"], "pc": st["pc"][:n // 2 + 1].tolist(),
             "g1": g1[LAGS].tolist(), "g3": g3[LAGS].tolist(), "stack3": stacking(files[n_therm:])}
 
 
-def main_two_per_site(work, ring):
-    """ring2: sl_equal from RingHost starts; lin2: sl_double from LinHost starts (see the module text)."""
+def main_two_per_site(work, ring, n=N, t_units=None, n_therm=None):
+    """ring2: sl_equal from RingHost starts; lin2: sl_double from LinHost starts (see the module text).
+    n, t_units, n_therm: chain length, time units and discarded frames (default: the N=100 protocol)."""
     exe = os.path.join(ROOT, "oracle", "bin", "efpol_port_ring" if ring else "efpol_port_lin")
-    nseed, t_units, interval, n_therm = (NSEED2, T2, INTERVAL, N_THERM2) if ring else (NSEEDL2, 10 * T, 10 * INTERVAL,
+    nseed, t_def, interval, therm_def = (NSEED2, T2, INTERVAL, N_THERM2) if ring else (NSEEDL2, 10 * T, 10 * INTERVAL,
                                                                                       N_THERM)
+    t_units = t_units or t_def
+    n_therm = n_therm or therm_def
     model = "sl_equal" if ring else "sl_double"
     host = elp.RingHost if ring else elp.LinHost
     dirs = []
     for k in range(nseed):
         d = os.path.join(work, f"k{k}")
         os.makedirs(d, exist_ok=True)
-        h = host(N=N, L=L, density=1.2, seed=k)
+        h = host(N=n, L=L, density=1.2, seed=k)
         h.write_tuv(os.path.join(d, "start.res"))
         h.close()
         dirs.append(d)
@@ -108,16 +116,16 @@ def main_two_per_site(work, ring):
                  for k in range(b, min(b + 8, nseed))]
         for p in procs:
             assert p.wait() == 0
-    out = {"L": L, "N": N, "density": 1.2, "model": f"{model}, at most two monomers per site", "time_units": t_units,
+    out = {"L": L, "N": n, "density": 1.2, "model": f"{model}, at most two monomers per site", "time_units": t_units,
            "interval": interval, "n_therm_frames": n_therm, "lags": LAGS,
-           "start": f"{host.__name__}(N=100, L=32, density=1.2, seed=k), efpol seed 2k+2, k = 0..{nseed - 1}",
+           "start": f"{host.__name__}(N={n}, L=32, density=1.2, seed=k), efpol seed 2k+2, k = 0..{nseed - 1}",
            "seeds": []}
     for k, d in enumerate(dirs):
-        row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(ring, frames_of(d, t_units, interval), n_therm)}
+        row = {"host_seed": k, "seed": 2 * k + 2, **seed_stats(ring, frames_of(d, t_units, interval, n), n_therm, n)}
         assert row["stack3"] == 0
         out["seeds"].append(row)
         print(k, row["rg2"], row["slrat"], flush=True)
-    with open(os.path.join(HERE, ("ring2" if ring else "lin2") + "_L32_N100.json"), "w") as f:
+    with open(os.path.join(HERE, ("ring2" if ring else "lin2") + f"_L32_N{n}.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
@@ -125,7 +133,11 @@ def main():
     args = sys.argv[1:]
     kind = args.pop(0) if args and args[0] in ("ring", "linear", "ring2", "lin2") else "ring"
     if kind in ("ring2", "lin2"):
-        return main_two_per_site(args[0] if args else tempfile.mkdtemp(), kind == "ring2")
+        opts = {}
+        while args and args[0].startswith("--"):
+            key, val = args.pop(0)[2:], int(args.pop(0))
+            opts[{"N": "n", "T": "t_units", "therm": "n_therm"}[key]] = val
+        return main_two_per_site(args[0] if args else tempfile.mkdtemp(), kind == "ring2", **opts)
     ring = kind == "ring"
     work = args[0] if args else tempfile.mkdtemp()
     t_units, interval = (T, INTERVAL) if ring else (10 * T, 10 * INTERVAL)

@@ -1,6 +1,6 @@
 """Equilibrium statistics of the GPU ring engine (ring melt L=32, N=100,
-density 1.2, 64 seeds run as one batch of 64 replicas, 100000 time units,
-frames every 1000, the first 41 discarded), for the reference schedule (32^3 tiles, cubic sub-lattice, 51
+density 1.2, 64 seeds run as one batch of 64 replicas, 200000 time units,
+frames every 1000, the first 101 discarded), for the reference schedule (32^3 tiles, cubic sub-lattice, 51
 iterations) and the dense schedule (16^3, dense, 13), against each other and
 against efpol (tests/golden/efpol_stats/ring_L32_N100.json, written by the
 script beside it).
@@ -20,7 +20,8 @@ agreement.  efpol restricted to at most two monomers per site
 starts) samples the uniform measure over the GPU state space (efpol's
 sl_equal table is symmetric: tests/test_efpol_table.py), which is the
 equilibrium of the balanced rates: there the statics must agree within
-3 sigma, for both schedules.
+3 sigma, for both schedules, and, on the dense schedule, at N = 50 as well
+(Rg^2(N); ring2_L32_N50.json).
 """
 import json
 import os
@@ -58,13 +59,16 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _run(name, rules=elp.RULES_REFERENCE):
-    if (name, rules) in _cache:
-        return _cache[(name, rules)]
+def _run(name, rules=elp.RULES_REFERENCE, ref=REF2):
+    """NSEED seeds of schedule `name` under the protocol of the efpol-2 fixture
+    `ref` (chain length, time units, frame interval, frames discarded)."""
+    n, t_units, interval, n_therm = ref["N"], ref["time_units"], ref["interval"], ref["n_therm_frames"]
+    if (name, rules, n) in _cache:
+        return _cache[(name, rules, n)]
     tile, sub, n_iter = {**SCHEDULES, **OTHER}[name]
     # the seeds as one batch of NSEED replicas (each bit-exact with its solo run)
-    _say(f"[statistics] {name} rules {rules}, seeds 0..{NSEED - 1} as one batch")
-    hosts = [elp.RingHost(N=N, L=L, density=REF["density"], seed=s) for s in range(NSEED)]
+    _say(f"[statistics] {name} rules {rules} N={n}, seeds 0..{NSEED - 1} as one batch")
+    hosts = [elp.RingHost(N=n, L=L, density=REF["density"], seed=s) for s in range(NSEED)]
     bat = elp.Batch("ring", [h.L for h in hosts], tile, sub, n_iter)
     bat.set_rules(rules)
     series = []
@@ -72,13 +76,13 @@ def _run(name, rules=elp.RULES_REFERENCE):
         bat.upload(r, h.lattice())
         h.load_shift(tile)
         # frames reduced on the device, replica by replica in place: no lattice download per frame
-        series.append(Series(True, h.L, h.n_pol, h.N, T // INTERVAL + 1, summary=True))
+        series.append(Series(True, h.L, h.n_pol, h.N, t_units // interval + 1, summary=True))
         series[r].add_summary(bat.frame_summary(r, h))
     tau = 0
-    for _ in range(T // INTERVAL):
-        org = np.stack([h.origins(tile, INTERVAL) for h in hosts], axis=1)
+    for _ in range(t_units // interval):
+        org = np.stack([h.origins(tile, interval) for h in hosts], axis=1)
         bat.run(org, tau0=tau, seeds=[h.seed for h in hosts], streams=[0] * NSEED, units_per_graph=16)
-        tau += INTERVAL
+        tau += interval
         for r, h in enumerate(hosts):
             series[r].add_summary(bat.frame_summary(r, h))
     res = []
@@ -88,12 +92,12 @@ def _run(name, rules=elp.RULES_REFERENCE):
         for c in range(h.n_pol):
             b = h.polymer(c)[1]
             stack3 += int(np.sum((b == 0) & (np.roll(b, 1) == 0)))
-        st = ser.statics(N_THERM)
-        g1, g3 = ser.msd(N_THERM, max(LAGS) + 1)
+        st = ser.statics(n_therm)
+        g1, g3 = ser.msd(n_therm, max(LAGS) + 1)
         res.append({"rg2": st["rg2"], "slrat": st["slrat"], "pc": st["pc"], "g1": g1[LAGS], "g3": g3[LAGS],
                     "stack3": stack3})
     bat.close()
-    _cache[(name, rules)] = res
+    _cache[(name, rules, n)] = res
     return res
 
 
@@ -234,3 +238,52 @@ def test_time_unit_factor_from_g3_predicts_g1(case):
     assert np.all(np.abs(dev) < 0.15), dev
     if rules == elp.RULES_REFERENCE:
         assert abs(f / (0.85 / 1.1) - 1.0) < 0.15, f
+
+
+CHAIN_LENGTHS = [50, 100]
+
+
+def _ref2(n):
+    """the efpol-2 fixture of chain length n (ring2_L32_N{n}.json)"""
+    return REF2 if n == REF2["N"] else json.load(open(os.path.join(HERE, "golden", "efpol_stats", f"ring2_L32_N{n}.json")))
+
+
+def test_rg2_of_n_equals_efpol_with_at_most_two_per_site():
+    """Rg^2(N), the stored-length ratio and P(s): the dense schedule with
+    balanced rates against efpol restricted to at most two monomers per site,
+    each from the GPU engine's own starts, at N = 50 and 100 (L=32,
+    density 1.2).  Every statistic
+    agrees within 3 sigma at every N, and Rg^2 grows with N on both sides
+    with the same exponent within its error.  Dynamics: at every N one
+    time-unit factor fitted on g3 predicts g1 within 15 %, and the factor
+    (a ratio of local move rates) is the same at every N within 15 %."""
+    rg, fs = {}, {}
+    for n in CHAIN_LENGTHS:
+        ref = _ref2(n)
+        g, e2 = _run("dense 16^3 13", elp.RULES_BALANCED, ref), ref["seeds"]
+        assert all(r["stack3"] == 0 for r in g) and all(r["stack3"] == 0 for r in e2)
+        pts = [s for s in S_POINTS if s <= n // 2]
+        zs = {key: _z([r[key] for r in g], [r[key] for r in e2]) for key in ("rg2", "slrat")}
+        zs["pc"] = _z([r["pc"][pts] for r in g], [np.asarray(r["pc"])[pts] for r in e2])
+        rg[n] = (_mean_se([r["rg2"] for r in g]), _mean_se([r["rg2"] for r in e2]))
+        _say(f"[Rg^2(N)] N={n}: GPU {rg[n][0][0]:.4f}+-{rg[n][0][1]:.4f} efpol-2 {rg[n][1][0]:.4f}+-{rg[n][1][1]:.4f}; "
+             f"z Rg^2 {zs['rg2']:.2f}, stored length {zs['slrat']:.2f}, P(s) at {pts} {np.round(zs['pc'], 2)}")
+        assert abs(zs["rg2"]) < 3 and abs(zs["slrat"]) < 3 and np.all(np.abs(zs["pc"]) < 3), (n, zs)
+        tau = np.asarray(LAGS, float)
+        g3g, g1g = _mean_se([r["g3"] for r in g])[0], _mean_se([r["g1"] for r in g])[0]
+        g3e, g1e = _mean_se([r["g3"] for r in e2])[0], _mean_se([r["g1"] for r in e2])[0]
+        fs[n] = _time_factor(tau, g3g, g3e)
+        inside, pred = _predict(tau, fs[n], g1e)
+        dev = g1g[inside] / pred[inside] - 1.0
+        _say(f"[Rg^2(N)] N={n}: time-unit factor {fs[n]:.3f} from g3, g1 deviation {np.round(dev, 3)}")
+        assert inside.sum() >= 3 and np.all(np.abs(dev) < 0.15), (n, dev)
+    assert max(fs.values()) / min(fs.values()) < 1.15, fs
+    # exponent of Rg^2 ~ N^a between the shortest and longest chains, with its propagated error
+    n0, n1 = min(CHAIN_LENGTHS), max(CHAIN_LENGTHS)
+    for side, k in (("GPU", 0), ("efpol-2", 1)):
+        (m1, s1), (m2, s2) = rg[n0][k], rg[n1][k]
+        a = np.log(m2 / m1) / np.log(n1 / n0)
+        rg[side] = (a, np.hypot(s1 / m1, s2 / m2) / np.log(n1 / n0))
+        _say(f"[Rg^2(N)] {side}: Rg^2 ~ N^{rg[side][0]:.4f} +- {rg[side][1]:.4f}")
+    assert rg["GPU"][0] > 0.5 and rg["efpol-2"][0] > 0.5
+    assert abs(rg["GPU"][0] - rg["efpol-2"][0]) < 3 * np.hypot(rg["GPU"][1], rg["efpol-2"][1])
