@@ -27,6 +27,7 @@ def _load():
         lib.elp_series_rg2.argtypes = [P, I, P]
         lib.elp_series_statics.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), P]
         lib.elp_series_msd.argtypes = [P, I, I, P, P]
+        lib.elp_series_msd_reference.argtypes = [P, I, I, P, P]
         lib.elp_series_new_summary.restype = P
         lib.elp_series_new_summary.argtypes = [I, I, I, I, I, I, I]
         lib.elp_series_add_summary.argtypes = [P, P, P, P, P, P, P]
@@ -122,8 +123,13 @@ class Series:
             raise ElpError("elp_series_statics failed")
         return {"rg2": rg2.value, "slrat": sl.value, "pc": pc}
 
-    def msd(self, n_therm, n_out):
+    def msd(self, n_therm, n_out, convention="drift"):
+        """g1, g3 at lags 0..n_out-1 frames from frame n_therm on.  convention
+        "drift" removes the whole displacement of the system mean (this
+        library's default); "reference" removes it as the reference analysis
+        does (g1 half of it, g3 1/sqrt(2): elp_series_msd_reference)."""
+        fn = {"drift": self.lib.elp_series_msd, "reference": self.lib.elp_series_msd_reference}[convention]
         g1, g3 = np.zeros(n_out), np.zeros(n_out)
-        if self.lib.elp_series_msd(self.h, int(n_therm), int(n_out), g1.ctypes.data, g3.ctypes.data):
+        if fn(self.h, int(n_therm), int(n_out), g1.ctypes.data, g3.ctypes.data):
             raise ElpError("elp_series_msd failed")
         return g1, g3

@@ -353,7 +353,8 @@ int elp_series_summary(const elp_series* s, int f, int32_t* start3, int64_t* sum
     return 0;
 }
 
-int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3) {
+/* w1, w3: the share of the system mean's displacement taken off g1 and g3 */
+static int series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3, double w1, double w3) {
     if (!s || n_therm < 0 || n_therm >= s->n_frames || n_out <= 0) return -1;
     const int nf = s->n_frames, nc = s->n_chain;
     const int mono = pol_size(s) / 2;
@@ -386,11 +387,13 @@ int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, doub
                 double x1, y1, z1, x2, y2, z2;
                 xyz_of(s->pos ? frame_chain(s, t, c) + 3 * mono : s->mid + 3 * ((size_t)t * nc + c), &x1, &y1, &z1);
                 xyz_of(s->pos ? frame_chain(s, t2, c) + 3 * mono : s->mid + 3 * ((size_t)t2 * nc + c), &x2, &y2, &z2);
-                const double mx = (x1 - x2) * r - dr[0], my = (y1 - y2) * r - dr[1], mz = (z1 - z2) * r - dr[2];
+                const double mx = (x1 - x2) * r - w1 * dr[0], my = (y1 - y2) * r - w1 * dr[1],
+                             mz = (z1 - z2) * r - w1 * dr[2];
                 a1 += mx * mx + my * my + mz * mz;
                 const double* c1 = cms + 3 * ((size_t)t * nc + c);
                 const double* c2 = cms + 3 * ((size_t)t2 * nc + c);
-                const double cx = c1[0] - c2[0] - dr[0], cy = c1[1] - c2[1] - dr[1], cz = c1[2] - c2[2] - dr[2];
+                const double cx = c1[0] - c2[0] - w3 * dr[0], cy = c1[1] - c2[1] - w3 * dr[1],
+                             cz = c1[2] - c2[2] - w3 * dr[2];
                 a3 += cx * cx + cy * cy + cz * cz;
                 num++;
             }
@@ -401,4 +404,16 @@ int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, doub
     free(cms);
     free(sys);
     return 0;
+}
+
+int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3) {
+    return series_msd(s, n_therm, n_out, g1, g3, 1.0, 1.0);
+}
+
+/* analysis/mode_compute.c:506-570 with the drift of analysis/mode_compute.c:244-251 and
+   analysis/mode_io.c:640-652: avg_pos holds the mean over chains of cms / sqrt(2); g3 subtracts it
+   from a centre-of-mass displacement in bond units, g1 from a middle-monomer displacement in
+   lattice units before dividing by 2 */
+int elp_series_msd_reference(const elp_series* s, int n_therm, int n_out, double* g1, double* g3) {
+    return series_msd(s, n_therm, n_out, g1, g3, 0.5, 1.0 / sqrt(2.0));
 }

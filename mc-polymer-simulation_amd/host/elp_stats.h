@@ -67,6 +67,17 @@ int elp_series_statics(const elp_series* s, int n_therm, double* rg2, double* sl
 int elp_series_msd(const elp_series* s, int n_therm, int n_out, double* g1, double* g3);
 
 /*
+ * The same displacements with the reference analysis's drift convention
+ * (analysis/mode_compute.c:244-251,506-570, analysis/mode_io.c:640-652): its
+ * avg_pos file holds the mean over chains of cms/sqrt(2), which g3 subtracts
+ * from a centre-of-mass displacement in bond units and g1 from a
+ * middle-monomer displacement in lattice units before dividing by 2, so g3
+ * removes 1/sqrt(2) of the system mean's displacement and g1 half of it.
+ * Reproduces the reference's g1/g3 files from the same frames.
+ */
+int elp_series_msd_reference(const elp_series* s, int n_therm, int n_out, double* g1, double* g3);
+
+/*
  * Summary mode: a series fed with per-chain reductions of each frame instead
  * of all monomer positions (what elp_ring_frame_summary of libelp.so returns):
  * per chain c its start site start3[3c..] in the box, the sum of its monomer

@@ -232,6 +232,9 @@ def test_msd_drift_convention_of_the_reference_differs_negligibly():
     assert np.allclose(msd(mid, 1.0), g1, rtol=1e-12) and np.allclose(msd(cms, 1.0), g3, rtol=1e-12)
     g1_ref, g3_ref = msd(mid, 0.5), msd(cms, 1 / np.sqrt(2))
     assert g3[1] > 0.05                                     # the chains moved
+    # the library's reference-convention entry point gives exactly these
+    r1, r3 = full.msd(0, nf - 1, convention="reference")
+    assert np.allclose(r1, g1_ref, rtol=1e-12) and np.allclose(r3, g3_ref, rtol=1e-12)
     # g3: the chains' mean displacement is the system's, so the cross term vanishes and the
     # reference's g3 exceeds this library's by exactly (1 - 1/sqrt(2))^2 x the system mean's MSD
     sys_msd = np.array([np.mean(np.sum((sysm[:nf - k] - sysm[k:]) ** 2, axis=1)) for k in range(nf - 1)])
