@@ -20,8 +20,8 @@ agreement.  efpol restricted to at most two monomers per site
 starts) samples the uniform measure over the GPU state space (efpol's
 sl_equal table is symmetric: tests/test_efpol_table.py), which is the
 equilibrium of the balanced rates: there the statics must agree within
-3 sigma, for both schedules, and, on the dense schedule, at N = 50 as well
-(Rg^2(N); ring2_L32_N50.json).
+3 sigma, for both schedules, and, on the dense schedule, at N = 50 and 200 as
+well (Rg^2(N); ring2_L32_N50.json, ring2_L32_N200.json).
 """
 import json
 import os
@@ -240,7 +240,7 @@ def test_time_unit_factor_from_g3_predicts_g1(case):
         assert abs(f / (0.85 / 1.1) - 1.0) < 0.15, f
 
 
-CHAIN_LENGTHS = [50, 100]
+CHAIN_LENGTHS = [50, 100, 200]
 
 
 def _ref2(n):
@@ -251,8 +251,9 @@ def _ref2(n):
 def test_rg2_of_n_equals_efpol_with_at_most_two_per_site():
     """Rg^2(N), the stored-length ratio and P(s): the dense schedule with
     balanced rates against efpol restricted to at most two monomers per site,
-    each from the GPU engine's own starts, at N = 50 and 100 (L=32,
-    density 1.2).  Every statistic
+    each from the GPU engine's own starts, at N = 50, 100 and 200 (L=32,
+    density 1.2; N = 200 runs 300000 time units and drops the first 151
+    frames, its rings relaxing about four times slower).  Every statistic
     agrees within 3 sigma at every N, and Rg^2 grows with N on both sides
     with the same exponent within its error.  Dynamics: at every N one
     time-unit factor fitted on g3 predicts g1 within 15 %, and the factor
