@@ -103,3 +103,37 @@ def test_lin_largest_box_bit_exact():
     want = replay_lin(lat0, h.L, tile, elp.SUBLAT_DENSE, 16, org, 0, h.seed)
     assert np.count_nonzero(want != lat0) > lat0.size // 4
     assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
+
+
+def test_lin_c3_full_length():
+    """C3 at its full length: linear L=128, N=500 (5033 chains), 10^5 time units
+    on the dense 16^3 schedule.  Size-independent checks at that length: every
+    chain intact (one start, one end, consistent bonds), the monomer count and
+    every chain's label sequence unchanged; the same units run as two halves
+    (the second resumed at time unit 50000) give the identical lattice; and the
+    run as replica 0 of a batch gives the identical lattice."""
+    print("[lin C3] L=128 N=500, 100000 time units, dense 16^3", flush=True)
+    tile = (16, 16, 16)
+    h = elp.LinHost(N=500, L=128, density=1.2, seed=5)
+    lat0 = h.lattice()
+    before = lin_label_sequences(lat0, h.L)
+    org = h.origins(tile, 100000)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    assert lin_site_invariants(got, h.L) == (5033 * 500, 5033)
+    assert lin_label_sequences(got, h.L) == before
+    print("  one run: invariants and label sequences kept", flush=True)
+    eng.upload(lat0)
+    eng.run(org[:50000], tau0=0, seed=h.seed)
+    eng.run(org[50000:], tau0=50000, seed=h.seed)
+    assert np.array_equal(eng.download(), got)
+    eng.close()
+    print("  two halves: identical", flush=True)
+    bat = elp.Batch("linear", [h.L], tile, elp.SUBLAT_DENSE, 16)
+    bat.upload(0, lat0)
+    bat.run(org[:, None, :], tau0=[0], seeds=[h.seed], streams=[0], units_per_graph=64)
+    assert np.array_equal(bat.download(0), got)
+    bat.close()
+    print("  as replica 0 of a batch: identical", flush=True)

@@ -228,3 +228,44 @@ def test_ring_compacted_variant_bit_exact():
     r = subprocess.run([sys.executable, "-c", _COMPACT_CHECK], env=env, capture_output=True, text=True, timeout=300)
     _say(r.stdout + r.stderr)
     assert r.returncode == 0
+
+
+def test_ring_c2_full_length():
+    """C2 at its full length: ring L=96, N=1000 (1062 rings), 10^5 time units on
+    the dense 16^3 schedule from seed 0's start.  Size-independent checks at
+    that length: every ring is intact, the monomer count and every ring's
+    cyclic label sequence are unchanged; the same units run as two halves (the
+    second resumed at time unit 50000) give the identical lattice; and the run
+    as replica 0 of a batch (beside a replica of seed 1) gives the identical
+    lattice."""
+    _say("[ring C2] L=96 N=1000, 100000 time units, dense 16^3")
+    tile = (16, 16, 16)
+    h = elp.RingHost(N=1000, L=96, density=1.2, seed=0)
+    lat0 = h.lattice()
+    before = ring_label_sequences(lat0, h.L)
+    h.load_shift(tile)
+    org = h.origins(tile, 100000)
+    eng = elp.Engine("ring", h.L, tile, elp.SUBLAT_DENSE, 13)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    got = eng.download()
+    assert site_invariants(got, h.L) == 1062 * 1000
+    assert ring_label_sequences(got, h.L) == before
+    _say("  one run: invariants and label sequences kept")
+    eng.upload(lat0)
+    eng.run(org[:50000], tau0=0, seed=h.seed)
+    eng.run(org[50000:], tau0=50000, seed=h.seed)
+    assert np.array_equal(eng.download(), got)
+    eng.close()
+    _say("  two halves: identical")
+    h1 = elp.RingHost(N=1000, L=96, density=1.2, seed=1)
+    h1.load_shift(tile)
+    org1 = h1.origins(tile, 100000)
+    bat = elp.Batch("ring", [h.L, h1.L], tile, elp.SUBLAT_DENSE, 13)
+    bat.upload(0, lat0)
+    bat.upload(1, h1.lattice())
+    bat.run(np.stack([org, org1], axis=1), tau0=[0, 0], seeds=[h.seed, h1.seed], streams=[0, 0], units_per_graph=64)
+    assert np.array_equal(bat.download(0), got)
+    assert site_invariants(bat.download(1), h1.L) == 1062 * 1000
+    bat.close()
+    _say("  as replica 0 of a batch: identical")
