@@ -229,13 +229,28 @@ def _rg2_now(host, eng):
     return float(np.mean(eng.ring_stats(host)[0]))
 
 
+P_WINDOW = (500, 2000)          # contour separations averaged for the contact-probability probe
+
+
+def _contact_now(host, eng):
+    """Contact probability P(s) averaged over s in P_WINDOW, from one frame reduced on the device
+    (elp_ring_frame_summary).  The compact start's rings meet far along their contour only at
+    their turns, so it rises as they crumple and swell."""
+    from elp.stats import Series
+    ser = Series(True, host.L, host.n_pol, host.N, 1, summary=True)
+    ser.add_summary(eng.ring_frame_summary(host))
+    pc = ser.statics(0)["pc"]
+    return float(np.mean(pc[P_WINDOW[0]:P_WINDOW[1] + 1]))
+
+
 def bench_learned(args):
     """Equilibration A/B: ring melt L=128, N=4000, density 1.2 (629 rings) from
     the compact start, plain engine (balanced rates, 16^3 dense, 13
     iterations) vs the learned proposal (16^3 cubic, 51 iterations, network
     trained on 1e6 moves harvested from an L=96 N=1000 melt run by the plain
     engine), each for the same wall-clock budget; Rg^2 sampled along the way.
-    value = wall-time ratio plain / learned to reach the Rg^2 both reach."""
+    value = wall-time ratio plain / learned to reach the Rg^2 both reach; the same ratio for the
+    contact probability at large contour separations is reported beside it."""
     import elp
     budget = args.budget
     tile = (16, 16, 16)
@@ -263,7 +278,7 @@ def bench_learned(args):
         eng.set_rules(elp.RULES_BALANCED)
         eng.upload(h.lattice())
         h.load_shift(tile)
-        pts = [(0.0, 0, _rg2_now(h, eng))]
+        pts = [(0.0, 0, _rg2_now(h, eng), _contact_now(h, eng))]
         tau, gpu_s, chunk = 0, 0.0, (2000 if kind == "plain" else 20)
         while gpu_s < budget:
             org = h.origins(tile, chunk)
@@ -277,20 +292,30 @@ def bench_learned(args):
                 lp_blocks = eng.launch_info()["blocks"]
             gpu_s += time.perf_counter() - t1
             tau += chunk
-            pts.append((gpu_s, tau, _rg2_now(h, eng)))
+            pts.append((gpu_s, tau, _rg2_now(h, eng), _contact_now(h, eng)))
             chunk = min(chunk * 2, 64000 if kind == "plain" else 640)
         curves[kind] = pts
         eng.close()
-    target = min(curves["plain"][-1][2], curves["learned"][-1][2])
-
-    def t_reach(pts):
-        for (ta, ua, ra), (tb, ub, rb) in zip(pts, pts[1:]):
-            if rb >= target:
-                f = (target - ra) / (rb - ra) if rb > ra else 1.0
-                return ta + f * (tb - ta), ua + f * (ub - ua)
+    def t_reach(pts, col, target, rising):
+        """GPU seconds and time units at which column col of the curve first reaches target
+        (linear interpolation between samples)"""
+        for a, b in zip(pts, pts[1:]):
+            if (b[col] >= target) if rising else (b[col] <= target):
+                f = (target - a[col]) / (b[col] - a[col]) if b[col] != a[col] else 1.0
+                return a[0] + f * (b[0] - a[0]), a[1] + f * (b[1] - a[1])
         return pts[-1][0], pts[-1][1]
 
-    (tp, up), (tl, ul) = t_reach(curves["plain"]), t_reach(curves["learned"])
+    def common(col):
+        """the level both arms reach: the statistic's direction from the compact start is that of the
+        plain arm (the longer run); the common level is the nearer of the two final values"""
+        rising = curves["plain"][-1][col] >= curves["plain"][0][col]
+        ends = (curves["plain"][-1][col], curves["learned"][-1][col])
+        target = min(ends) if rising else max(ends)
+        return target, t_reach(curves["plain"], col, target, rising), t_reach(curves["learned"], col, target, rising)
+
+    # Rg^2 (rises as the compact rings swell) and the contact probability at large separations
+    target, (tp, up), (tl, ul) = common(2)
+    ptarget, (tpp, upp), (tlp, ulp) = common(3)
     # MFMA roofline of the learned kernel: per proposal two network evaluations of
     # 144*64 + 64*64 + 64*11 multiply-adds; one launch = every 16^3 tile x 64 slots
     # x 51 iterations x 4 substeps
@@ -315,7 +340,13 @@ def bench_learned(args):
             "plain_reach": {"seconds": tp, "time_units": up},
             "learned_reach": {"seconds": tl, "time_units": ul},
             "time_units_ratio_plain_over_learned": up / ul if ul else None,
-            "curves": {k: [[round(a, 3), b, round(c, 4)] for a, b, c in v] for k, v in curves.items()},
+            "contact_probability": {
+                "statistic": f"mean P(s) over s = {P_WINDOW[0]}..{P_WINDOW[1]}", "target": ptarget,
+                "plain_reach": {"seconds": tpp, "time_units": upp}, "learned_reach": {"seconds": tlp, "time_units": ulp},
+                "wall_time_ratio_plain_over_learned": tpp / tlp if tlp else None,
+                "time_units_ratio_plain_over_learned": upp / ulp if ulp else None},
+            "curves": {k: [[round(a, 3), b, round(c, 4), float(f"{d:.4g}")] for a, b, c, d in v] for k, v in curves.items()},
+            "curve_columns": ["GPU seconds", "time units", "mean Rg^2", "contact probability (statistic above)"],
             "parallelism": "single GPU",
         },
         "roofline": {
