@@ -1118,7 +1118,8 @@ __global__ __launch_bounds__(256, 2) void lp_ring_time_unit2(uint8_t* __restrict
             // lane c applies m to the tile (S -> S')
             int ok = 0, rv = -1, nch = 0, flip = -1, chi[3] = {0, 0, 0};
             uint32_t cho[3] = {0, 0, 0};
-            if (g == 0) {
+            // (m < 0 only if no interval held u, which non-finite logits would cause: no move)
+            if (g == 0 && m >= 0) {
                 ok = lp_move_tile(M.tile, st, su, sv, m, &rv, chi, cho, &nch, &flip);
                 lp_occ_flip(M.occ, flip);
             }
