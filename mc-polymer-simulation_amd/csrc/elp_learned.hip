@@ -782,13 +782,14 @@ struct LpMc2Shared {
     uint8_t tile[4096];
 };
 
-// input feature of layer-1 k-slot (g, kk, j), or -1 for a zero slot
-__device__ __forceinline__ int lp_feat1(int g, int kk, int j) {
-    const int r = 4 * kk + g;
+// input feature j of feature row r (rows of eight slots), or -1 for a zero slot
+__device__ __forceinline__ int lp_feat_row(int r, int j) {
     if (r < 25) return j < 5 ? 5 * r + j : -1;     // patch site t = j of row r = u + 5v
     if (r < 27) return 125 + 8 * (r - 25) + j;     // bond one-hot
     return j < 3 ? 141 + j : -1;                   // stored length, partner's, constant
 }
+// input feature of layer-1 k-slot (g, kk, j): feature j of row 4kk + g
+__device__ __forceinline__ int lp_feat1(int g, int kk, int j) { return lp_feat_row(4 * kk + g, j); }
 // hidden unit of layer-2/3 k-slot (g, kk, j)
 __device__ __forceinline__ int lp_hid(int g, int kk, int j) { return 16 * (2 * kk + (j >> 2)) + 4 * g + (j & 3); }
 
@@ -801,20 +802,21 @@ __device__ __forceinline__ uint32_t lp_occ_word(const uint8_t* tile, int i) {
     return 0xFF0000FFu | (o << 8);
 }
 
-// Bitmap offset of patch row 4kk + g from the patch's row (u, v) = (-2, -2)
-// (row 24 where 4kk + g > 24: those lanes replace it by their tail row).
-__device__ __forceinline__ int lp_row_offset(int g, int kk) {
-    const int r = (4 * kk + g < 25) ? 4 * kk + g : 24;
+// Bitmap offset of patch row r from the patch's row (u, v) = (-2, -2) (row
+// 24 for the tail rows r > 24: read, then replaced by the tail features).
+__device__ __forceinline__ int lp_row_offset_r(int r) {
+    r = r < 25 ? r : 24;
     return LP_OCCW * (r / 5) + r % 5;
 }
+__device__ __forceinline__ int lp_row_offset(int g, int kk) { return lp_row_offset_r(4 * kk + g); }
 
-// Features of layer-1 k-step kk in lane group g as eight bits: the patch row
-// from the bitmap (occ0: index of the slot's row (u, v) = (-2, -2), tsh: bit
-// of t = -2), or the tail row (g >= 1 at the last k-step).
+// Eight features of a row as bits: a patch row from the bitmap (occ0: index of
+// the slot's row (u, v) = (-2, -2), rowoff: the row's offset, tsh: bit of
+// t = -2), or a tail row's bits.
 __device__ __forceinline__ uint32_t lp_row_bits(const uint32_t* occ, int occ0, int rowoff, int tsh, uint32_t tail,
-                                                int g, int kk) {
+                                                bool isTail) {
     const uint32_t bits = (occ[occ0 + rowoff] >> tsh) & 0x1fu;
-    return (kk == LP_K1 - 1 && g != 0) ? tail : bits;
+    return isTail ? tail : bits;
 }
 
 __device__ __forceinline__ uint32_t lp_pack2(float lo, float hi) {
@@ -866,7 +868,7 @@ __device__ __forceinline__ f32x4 lp_forward_t(const LpMc2Shared& M, const bf16x8
     for (int rb = 0; rb < 4; rb++) d1[rb] = f32x4{0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < LP_K1; kk++) {
-        const bf16x8 b = lds_frag(M.x8[lp_row_bits(M.occ, occ0, rowoff[kk], tsh, tail, g, kk)]);
+        const bf16x8 b = lds_frag(M.x8[lp_row_bits(M.occ, occ0, rowoff[kk], tsh, tail, kk == LP_K1 - 1 && g != 0)]);
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) d1[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[rb][kk], b, d1[rb], 0, 0, 0);
     }
@@ -1216,7 +1218,7 @@ __global__ __launch_bounds__(64) void lp_tile_check_kernel(const uint8_t* __rest
     for (int g = 0; g < 4; g++) {
         const uint32_t tail = lp_tail_bits(tile, t, u, v, g);
         for (int kk = 0; kk < LP_K1; kk++) {
-            const uint32_t bits = lp_row_bits(occ, occ0, lp_row_offset(g, kk), tsh, tail, g, kk);
+            const uint32_t bits = lp_row_bits(occ, occ0, lp_row_offset(g, kk), tsh, tail, kk == LP_K1 - 1 && g != 0);
             for (int j = 0; j < 8; j++) {
                 const int f = lp_feat1(g, kk, j);
                 const uint32_t want = f >= 0 ? (x[f] != 0 ? 1u : 0u) : 0u;
