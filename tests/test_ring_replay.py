@@ -87,6 +87,43 @@ def test_dense_sublattice_neighbourhoods_disjoint():
     del bonds
 
 
+def test_no_denser_sublattice_than_dense():
+    """The dense schedule's sub-lattice is the densest one made of cosets of
+    4Z^3 whose move neighbourhoods stay disjoint: among all subgroups of
+    (Z/4)^3, the ones whose nonzero elements (every lift into [-4, 7]^3) avoid
+    the differences of two neighbourhood offsets have 1, 2 or 4 elements, and
+    the one with 4 is <(1,3,2)>."""
+    nb = {((b & 1) - (b >> 3), ((b >> 1) & 1) - (b >> 3), ((b >> 2) & 1) - (b >> 3))
+          for b in range(16) if b not in (0, 3, 12, 15)} | {(0, 0, 0)}
+    diff = {(a[0] - b[0], a[1] - b[1], a[2] - b[2]) for a in nb for b in nb}
+    assert len(diff) == 55
+    pts = [(t, u, v) for t in range(4) for u in range(4) for v in range(4)]
+
+    def join(group, g):
+        # the subgroup generated by group and g (abelian: group + <g>)
+        return frozenset(((x[0] + k * g[0]) % 4, (x[1] + k * g[1]) % 4, (x[2] + k * g[2]) % 4)
+                         for x in group for k in range(4))
+
+    def span(gens):
+        group = frozenset({(0, 0, 0)})
+        for g in gens:
+            group = join(group, g)
+        return group
+
+    # every subgroup, built up one generator at a time from the trivial one
+    groups, frontier = set(), {span([])}
+    while frontier:
+        groups |= frontier
+        frontier = {join(g, x) for g in frontier for x in pts if x not in g} - groups
+
+    def independent(group):
+        return all((x[0] + 4 * i, x[1] + 4 * j, x[2] + 4 * k) not in diff
+                   for x in group if x != (0, 0, 0) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1))
+    good = [g for g in groups if independent(g)]
+    assert sorted(len(g) for g in good) == [1, 2, 4]
+    assert max(good, key=len) == span([(1, 3, 2)])
+
+
 @pytest.mark.parametrize("tile,sublat,n_iter", [((16, 16, 16), 0, 51), ((16, 16, 16), 1, 13)])
 def test_replay_preserves_rings_and_labels(tile, sublat, n_iter):
     h, lat = _state(100, 32, 2)
