@@ -37,6 +37,13 @@ __device__ __forceinline__ uint4 philox4x32_10_lanes(uint4 c, uint32_t k0, uint3
     return c;
 }
 
+// Work without lattice dependence that a move runs right after issuing its
+// last dependent LDS read, so that it fills the wait for that read
+// (NoFill: none).
+struct NoFill {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 __device__ __forceinline__ uint32_t word_of(const uint4& w, int j) {
     return j == 0 ? w.x : j == 1 ? w.y : j == 2 ? w.z : w.w;
 }

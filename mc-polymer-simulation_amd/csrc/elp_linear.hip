@@ -429,12 +429,13 @@ __device__ __forceinline__ uint32_t lin_trans_move_interior(uint16_t* lat, uint3
 // no occupied chain-end site takes the two-case form on one scalar branch.
 // Both forms read the table word of the chosen bond (bondS, r & 3), which is
 // read before the branch so that the read and the test overlap.
-template <class G>
-__device__ __forceinline__ uint32_t lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r) {
+template <class G, class F = NoFill>
+__device__ __forceinline__ uint32_t lin_trans_move2(uint16_t* lat, uint32_t P, uint32_t r, const F& fill = F{}) {
     const uint32_t a = G::addr(P);
     const uint32_t w = lat[a];
     const uint32_t bondS = ((r >> 2) & 1u) ? (w >> 4) & 0xfu : w & 0xfu;
     const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
+    fill();
     const bool occEnd = ((w & 0xffu) != 0u) & (((w & 0xfu) == 0u) | ((w & 0xf0u) == 0u));
     if (__ballot(occEnd) != 0ull) return lin_trans_move_ends<G>(lat, P, r, a, w, wp);
     return lin_trans_move_interior<G>(lat, P, r, a, w, wp);
@@ -445,6 +446,7 @@ __device__ __forceinline__ uint32_t lin_trans_move2(uint16_t* lat, uint32_t P, u
 struct LinMoves {
     using site_t = uint16_t;
     static constexpr int NSUB = 2;
+    static constexpr bool HAS_RULES = false;
     static constexpr int EXTRA = 4 * kFoldPairs;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint16_t* lat, int tid) {
@@ -453,10 +455,11 @@ struct LinMoves {
             reinterpret_cast<uint32_t*>(lat + G::NSITE)[tid] = (G::delta(pair & 0xfu) & 0xFFFFFFu) | (pair << 24);
         }
     }
-    template <class G>
-    __device__ static __forceinline__ uint32_t move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t) {
-        if (s == 0) return lin_diffuse_sl2<G>(lat, P);
-        return lin_trans_move2<G>(lat, P, r);
+    template <class G, class F = NoFill>
+    __device__ static __forceinline__ uint32_t move(uint16_t* lat, uint32_t P, int s, uint32_t r, uint32_t,
+                                                    const F& fill = F{}) {
+        if (s == 0) { fill(); return lin_diffuse_sl2<G>(lat, P); }
+        return lin_trans_move2<G>(lat, P, r, fill);
     }
 };
 

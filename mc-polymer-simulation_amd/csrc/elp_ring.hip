@@ -382,8 +382,9 @@ __device__ __forceinline__ uint32_t diffuse_partner(const DiffuseParts& d, uint3
     return bitsel(d.m, ns | d.partnerTo, ns & 0x1fu);
 }
 
-template <class G, bool TABLE = false>
-__device__ __forceinline__ uint32_t trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules) {
+template <class G, bool TABLE = false, class F = NoFill>
+__device__ __forceinline__ uint32_t trans_back2(uint8_t* lat, uint32_t P, uint32_t r, uint32_t rules,
+                                                const F& fill = F{}) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -396,6 +397,7 @@ __device__ __forceinline__ uint32_t trans_back2(uint8_t* lat, uint32_t P, uint32
     const uint32_t newNext = cc & 0xfu;
     const uint32_t wa = G::addr(Pw);
     uint32_t ws = lat[wa];
+    fill();
     // what the site and source fix, computed while the last reads are in
     // flight (the empty statement keeps it ahead of the wait)
     const BackParts bp = back_parts(s, ss);
@@ -448,8 +450,8 @@ __device__ __forceinline__ uint32_t diffuse_sl2(uint8_t* lat, uint32_t P) {
 // and TransBack its fold site Pn + srcNext, one read of the drawn one,
 // beside the combine code.  The outcome and the writes are those of
 // trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
-template <class G>
-__device__ __forceinline__ uint32_t balanced_move(uint8_t* lat, uint32_t P, uint32_t r) {
+template <class G, class F = NoFill>
+__device__ __forceinline__ uint32_t balanced_move(uint8_t* lat, uint32_t P, uint32_t r, const F& fill = F{}) {
     const uint2* tab = ring_tab<G>(lat);
     const uint32_t ty = r >> 6;
     const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
@@ -466,6 +468,7 @@ __device__ __forceinline__ uint32_t balanced_move(uint8_t* lat, uint32_t P, uint
     const uint32_t X = isTB ? Pn + G::delta(ns & 0xfu) : P + wf;
     const uint32_t xa = G::addr(X);
     const uint32_t x = lat[xa];
+    fill();
     const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u, xsl = x & 0x40u;
     const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4, newNext = cc & 0xfu;
     // TransForw: an empty target next to both ends, stored length at one of them
@@ -501,6 +504,7 @@ template <bool PAIRS>
 struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
+    static constexpr bool HAS_RULES = true;
     static constexpr int EXTRA = PAIRS ? kRingTabTotal : 0;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {
@@ -513,24 +517,27 @@ struct RingMovesT {
                 ring_combine_tab<G>(lat)[i] = (uint8_t)combine_code((uint32_t)i >> 4, (uint32_t)i & 0xfu);
         }
     }
-    template <class G>
-    __device__ static __forceinline__ uint32_t move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules) {
+    template <class G, class F = NoFill>
+    __device__ static __forceinline__ uint32_t move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules,
+                                                    const F& fill = F{}) {
         if (rules) {
             // balanced rates: every slot draws its move from bits 6-7 of its
             // byte (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the
             // proportions of the reference sequence), so each substep is a
             // symmetric kernel and preserves the uniform measure by itself
             if constexpr (PAIRS) {
-                return balanced_move<G>(lat, P, r);
+                return balanced_move<G>(lat, P, r, fill);
             } else {
+                fill();
                 const uint32_t ty = r >> 6;
                 if (ty < 2u) return trans_forw2<G, PAIRS>(lat, P, r);
                 if (ty == 2u) return diffuse_sl2<G>(lat, P);
                 return trans_back2<G, PAIRS>(lat, P, r, rules);
             }
         }
-        if (s == 1) return diffuse_sl2<G>(lat, P);
-        if (s == 3) return trans_back2<G, PAIRS>(lat, P, r, rules);
+        if (s == 1) { fill(); return diffuse_sl2<G>(lat, P); }
+        if (s == 3) return trans_back2<G, PAIRS>(lat, P, r, rules, fill);
+        fill();
         return trans_forw2<G, PAIRS>(lat, P, r);
     }
 };

@@ -7,7 +7,8 @@
 // tile back in place.  The move set M supplies the site type, the number of
 // substeps per iteration and the move of substep s, which returns its
 // outcome (kMoveEmpty ... kMoveAccepted):
-//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules),
+//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules), M::HAS_RULES
+// (whether move reads rules; the loop is then instantiated per rule set),
 // and fills any LDS tables it keeps after the tile: M::init<G>(lds, tid).
 //
 // Randomness (Philox4x32-10, key = (seed, stream)):
@@ -19,6 +20,7 @@
 #pragma once
 #include "elp_device.h"
 #include "elp_kernels.h"
+#include <type_traits>
 
 namespace elp {
 
@@ -135,6 +137,14 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
     const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
 
+    // the random words of the first group of four iterations, drawn beside
+    // the tile load (they do not depend on the lattice)
+    uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, 0u), p.seed, p.stream);
+    uint4 sw[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++)
+        sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, (uint32_t)(tid + k * THREADS), 0u), p.seed, p.stream);
+
     // HBM -> LDS (periodic wrap; origins are < one tile edge, so one subtraction suffices)
     tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
 
@@ -156,37 +166,69 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     __syncthreads();
     uint32_t nEmpty = 0, nFace = 0, nAcc = 0;   // COUNT: outcomes of this thread's attempts
 
-    for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
-        const uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
-        uint4 sw[SPT];
+    // The move rates as a compile-time constant of the loop, so no substep
+    // branches on them.  Each group of four iterations draws the next
+    // group's words in the last substep of its first iteration, beside that
+    // substep's dependent LDS reads (the draws need no lattice data); the
+    // last group draws one unused set.
+    auto iterate = [&](auto rulesC) {
+        constexpr uint32_t RULES = decltype(rulesC)::value;
+        for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+            uint4 owNext = make_uint4(0u, 0u, 0u, 0u), swNext[SPT];
 #pragma unroll
-        for (int k = 0; k < SPT; k++)
-            sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed, p.stream);
-        const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
-        // unrolled, so that word j of each Philox output is a fixed register
+            for (int k = 0; k < SPT; k++) swNext[k] = owNext;
+            const int nj = (p.n_iter - it0) < 4 ? (p.n_iter - it0) : 4;
+            // unrolled, so that word j of each Philox output is a fixed register
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (j >= nj) break;
-            const uint32_t offw = word_of(ow, j);
+            for (int j = 0; j < 4; j++) {
+                if (j >= nj) break;
+                const uint32_t offw = word_of(ow, j);
 #pragma unroll
-            for (int s = 0; s < M::NSUB; s++) {
-                const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
-                const uint32_t cu = (c & 3u) << G::SU, cv = (c >> 2) << G::SV;
+                for (int s = 0; s < M::NSUB; s++) {
+                    // the next group's words; the empty statement keeps them
+                    // in the move's read block (the compiler would otherwise
+                    // sink them to the barrier, past the move's writes)
+                    auto draw = [&] {
+                        const uint32_t grp = (uint32_t)(it0 >> 2) + 1u;
+                        owNext = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, grp), p.seed, p.stream);
 #pragma unroll
-                for (int k = 0; k < SPT; k++) {
-                    uint32_t P;
-                    if (DENSE) P = ((cu + ju[k]) & (3u << G::SU)) | (vb[k] ^ cv);
-                    else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                    const uint32_t out = M::template move<G>(tile, P, s, (word_of(sw[k], j) >> (8 * s)) & 0xffu, p.rules);
-                    if constexpr (COUNT) {
-                        nEmpty += out == kMoveEmpty;
-                        nFace += out == kMoveFace;
-                        nAcc += out == kMoveAccepted;
+                        for (int k = 0; k < SPT; k++) {
+                            swNext[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, slot[k], grp), p.seed, p.stream);
+                            asm volatile("" ::"v"(swNext[k].x), "v"(swNext[k].y), "v"(swNext[k].z), "v"(swNext[k].w));
+                        }
+                        asm volatile("" ::"s"(owNext.x), "s"(owNext.y), "s"(owNext.z), "s"(owNext.w));
+                    };
+                    const uint32_t c = (offw >> (8 * s)) & (DENSE ? 0xfu : 0x3fu);
+                    const uint32_t cu = (c & 3u) << G::SU, cv = (c >> 2) << G::SV;
+#pragma unroll
+                    for (int k = 0; k < SPT; k++) {
+                        uint32_t P;
+                        if (DENSE) P = ((cu + ju[k]) & (3u << G::SU)) | (vb[k] ^ cv);
+                        else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
+                        const uint32_t rk = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
+                        // the last slot of the first iteration's last substep draws
+                        const uint32_t out = (j == 0 && s == M::NSUB - 1 && k == SPT - 1)
+                                                 ? M::template move<G>(tile, P, s, rk, RULES, draw)
+                                                 : M::template move<G>(tile, P, s, rk, RULES);
+                        if constexpr (COUNT) {
+                            nEmpty += out == kMoveEmpty;
+                            nFace += out == kMoveFace;
+                            nAcc += out == kMoveAccepted;
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
+            ow = owNext;
+#pragma unroll
+            for (int k = 0; k < SPT; k++) sw[k] = swNext[k];
         }
+    };
+    if constexpr (M::HAS_RULES) {
+        if (p.rules) iterate(std::integral_constant<uint32_t, 1u>{});
+        else iterate(std::integral_constant<uint32_t, 0u>{});
+    } else {
+        iterate(std::integral_constant<uint32_t, 0u>{});
     }
 
     // LDS -> HBM.  The thread index passes through an empty statement, so the
