@@ -490,7 +490,7 @@ def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, war
         raise RuntimeError("the counting kernels moved differently from the plain ones")
     info = eng.launch_info()
     eng.close()
-    a = c["attempts"]
+    a = max(c["attempts"], 1)   # a run of zero iterations (fixed-cost measurement) attempts nothing
     launch_s = kernel_ms * 1e-3 / steps
     return {
         "tile": list(tile), "sublattice": sublattice, "n_iter": n_iter, "threads": info["threads"],

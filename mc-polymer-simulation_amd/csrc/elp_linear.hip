@@ -287,9 +287,9 @@ __device__ __forceinline__ uint32_t lin_diffuse_sl2(uint16_t* lat, uint32_t P) {
 // the split comes from the LDS table after the tile: word 4 bondS + (r & 3)
 // holds fold_pair in bits 24-31 above the packed offset of its first bond
 // (nPrev) in bits 0-23, so the split's X1 is one add of that word.
-template <class G>
+template <class G, class F = NoFill>
 __device__ __forceinline__ uint32_t lin_trans_move_ends(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a, uint32_t w,
-                                                        uint32_t wp) {
+                                                        uint32_t wp, const F& fill = F{}) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     // stored-length cases: split the chosen bond, or grow the end
@@ -317,6 +317,7 @@ __device__ __forceinline__ uint32_t lin_trans_move_ends(uint16_t* lat, uint32_t 
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     const uint32_t x1 = lat[a1];
     const uint32_t x2 = lat[a2];
+    fill();
     // The four cases as selects and one guarded write block.
     const bool occ = (w & 0xffu) != 0u;
     const bool gate = ((r >> 4) & 1u) == 0u;            // cases without stored length
@@ -374,9 +375,9 @@ __device__ __forceinline__ uint32_t lin_trans_move_ends(uint16_t* lat, uint32_t 
 // length) and the fold (none) can happen, the split's bond is never 0 and
 // the fold needs no end test.  Same outcome as lin_trans_move_ends for such a
 // wave; empty lanes read in-tile sites and write nothing.
-template <class G>
+template <class G, class F = NoFill>
 __device__ __forceinline__ uint32_t lin_trans_move_interior(uint16_t* lat, uint32_t P, uint32_t r, uint32_t a,
-                                                            uint32_t w, uint32_t wp) {
+                                                            uint32_t w, uint32_t wp, const F& fill = F{}) {
     const uint32_t next = w & 0xfu, prev = (w >> 4) & 0xfu;
     const bool sl = (w & kSL) != 0u;
     const uint32_t pbS = (r >> 2) & 1u;
@@ -390,6 +391,7 @@ __device__ __forceinline__ uint32_t lin_trans_move_interior(uint16_t* lat, uint3
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     const uint32_t x1 = lat[a1];
     const uint32_t x2 = lat[a2];
+    fill();
     const bool occ = (w & 0xffu) != 0u;
     const bool gate = ((r >> 4) & 1u) == 0u;
     const bool inTile = !G::out(X2 | X1);
@@ -435,10 +437,9 @@ __device__ __forceinline__ uint32_t lin_trans_move2(uint16_t* lat, uint32_t P, u
     const uint32_t w = lat[a];
     const uint32_t bondS = ((r >> 2) & 1u) ? (w >> 4) & 0xfu : w & 0xfu;
     const uint32_t wp = reinterpret_cast<const uint32_t*>(lat + G::NSITE)[(bondS << 2) | (r & 3u)];
-    fill();
     const bool occEnd = ((w & 0xffu) != 0u) & (((w & 0xfu) == 0u) | ((w & 0xf0u) == 0u));
-    if (__ballot(occEnd) != 0ull) return lin_trans_move_ends<G>(lat, P, r, a, w, wp);
-    return lin_trans_move_interior<G>(lat, P, r, a, w, wp);
+    if (__ballot(occEnd) != 0ull) return lin_trans_move_ends<G>(lat, P, r, a, w, wp, fill);
+    return lin_trans_move_interior<G>(lat, P, r, a, w, wp, fill);
 }
 
 // linpol's substep sequence {DiffuseSL, TransMove} (gpupol/linpol.cu:483-490)
@@ -447,6 +448,9 @@ struct LinMoves {
     using site_t = uint16_t;
     static constexpr int NSUB = 2;
     static constexpr bool HAS_RULES = false;
+    // the linear TransMove is closer to VALU-bound: the early draw (one unused
+    // group per launch) measured 0.6 % slower than drawing at each group's start
+    static constexpr bool DRAW_AHEAD = false;
     static constexpr int EXTRA = 4 * kFoldPairs;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint16_t* lat, int tid) {

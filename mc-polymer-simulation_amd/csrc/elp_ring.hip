@@ -505,6 +505,7 @@ struct RingMovesT {
     using site_t = uint8_t;
     static constexpr int NSUB = 4;
     static constexpr bool HAS_RULES = true;
+    static constexpr bool DRAW_AHEAD = true;
     static constexpr int EXTRA = PAIRS ? kRingTabTotal : 0;   // LDS bytes after the tile
     template <class G>
     __device__ static __forceinline__ void init(uint8_t* lat, int tid) {

@@ -7,8 +7,11 @@
 // tile back in place.  The move set M supplies the site type, the number of
 // substeps per iteration and the move of substep s, which returns its
 // outcome (kMoveEmpty ... kMoveAccepted):
-//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules), M::HAS_RULES
-// (whether move reads rules; the loop is then instantiated per rule set),
+//   M::site_t, M::NSUB, M::move<G>(lds, P, s, r, rules[, fill]),
+//   M::HAS_RULES (whether move reads rules; the loop is then instantiated
+//   per rule set), M::DRAW_AHEAD (whether the next group of iterations'
+//   random words are drawn inside a move, by the callable fill that the
+//   move runs after issuing its last dependent LDS read),
 // and fills any LDS tables it keeps after the tile: M::init<G>(lds, tid).
 //
 // Randomness (Philox4x32-10, key = (seed, stream)):
@@ -166,14 +169,24 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     __syncthreads();
     uint32_t nEmpty = 0, nFace = 0, nAcc = 0;   // COUNT: outcomes of this thread's attempts
 
-    // The move rates as a compile-time constant of the loop, so no substep
-    // branches on them.  Each group of four iterations draws the next
-    // group's words in the last substep of its first iteration, beside that
-    // substep's dependent LDS reads (the draws need no lattice data); the
-    // last group draws one unused set.
+    // The move rates are a compile-time constant of the loop, so no substep
+    // branches on them.  With M::DRAW_AHEAD, each group of four iterations
+    // draws the next group's words in the last substep of its first
+    // iteration, after that substep's last dependent LDS read (the draws need
+    // no lattice data, so they fill the wait; the last group draws one unused
+    // set); otherwise each group draws its own words at its start.
     auto iterate = [&](auto rulesC) {
         constexpr uint32_t RULES = decltype(rulesC)::value;
         for (int it0 = 0; it0 < p.n_iter; it0 += 4) {
+            if constexpr (!M::DRAW_AHEAD) {
+                if (it0 > 0) {
+                    ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, (uint32_t)(it0 >> 2)), p.seed, p.stream);
+#pragma unroll
+                    for (int k = 0; k < SPT; k++)
+                        sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, slot[k], (uint32_t)(it0 >> 2)), p.seed,
+                                                    p.stream);
+                }
+            }
             uint4 owNext = make_uint4(0u, 0u, 0u, 0u), swNext[SPT];
 #pragma unroll
             for (int k = 0; k < SPT; k++) swNext[k] = owNext;
@@ -185,7 +198,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                 const uint32_t offw = word_of(ow, j);
 #pragma unroll
                 for (int s = 0; s < M::NSUB; s++) {
-                    // the next group's words; the empty statement keeps them
+                    // the next group's words; the empty statements keep them
                     // in the move's read block (the compiler would otherwise
                     // sink them to the barrier, past the move's writes)
                     auto draw = [&] {
@@ -207,7 +220,7 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                         else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
                         const uint32_t rk = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
                         // the last slot of the first iteration's last substep draws
-                        const uint32_t out = (j == 0 && s == M::NSUB - 1 && k == SPT - 1)
+                        const uint32_t out = (M::DRAW_AHEAD && j == 0 && s == M::NSUB - 1 && k == SPT - 1)
                                                  ? M::template move<G>(tile, P, s, rk, RULES, draw)
                                                  : M::template move<G>(tile, P, s, rk, RULES);
                         if constexpr (COUNT) {
@@ -219,9 +232,11 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                     __syncthreads();
                 }
             }
-            ow = owNext;
+            if constexpr (M::DRAW_AHEAD) {
+                ow = owNext;
 #pragma unroll
-            for (int k = 0; k < SPT; k++) sw[k] = swNext[k];
+                for (int k = 0; k < SPT; k++) sw[k] = swNext[k];
+            }
         }
     };
     if constexpr (M::HAS_RULES) {
