@@ -48,6 +48,18 @@ __device__ __forceinline__ uint32_t word_of(const uint4& w, int j) {
     return j == 0 ? w.x : j == 1 ? w.y : j == 2 ? w.z : w.w;
 }
 
+// floor(n / d) for n < 2^22 and 1 <= d < 2^22 (tile indices and tile
+// counts: a box holds fewer than 2^31 sites and a tile at least 2^11).  The
+// float quotient (n and d exact in float, v_rcp_f32 within one ulp) is within
+// 0.75 of n / d, so its truncation is off by at most one, and one
+// correction step by the remainder makes it exact.  About ten instructions
+// instead of the generic 32-bit division's thirty.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t n, uint32_t d) {
+    const uint32_t q = (uint32_t)((float)n * __builtin_amdgcn_rcpf((float)d));
+    const int32_t r = (int32_t)(n - q * d);
+    return r < 0 ? q - 1u : (r >= (int32_t)d ? q + 1u : q);
+}
+
 // Counter word 2 of the tile-uniform site-offset stream; slot streams use
 // the slot index (always < 2^31) there instead.
 constexpr uint32_t kOffsetStream = 0xFFFFFFFFu;

@@ -122,6 +122,43 @@ __device__ __forceinline__ void tile_copy(T* __restrict__ glat, T* tile, const U
     }
 }
 
+// HBM -> LDS with lattice-independent work between issuing the loads and
+// writing LDS: in the plane-stepping layout (THREADS a multiple of TT*TU)
+// the loads go to registers first, `between` runs while they are in flight,
+// and the values are written to LDS after it; in the other layouts `between`
+// runs after the copy.
+template <int TT, int TU, int TV, int THREADS, int PAD, class T, class F>
+__device__ __forceinline__ void tile_load(T* __restrict__ glat, T* tile, const UnitArgs& p, int gt0, int gu0, int gv0,
+                                          int tid, const F& between) {
+    using G = PTile<TT, TU, TV, PAD>;
+    if constexpr (THREADS % (TT * TU) == 0) {
+        constexpr int VSTEP = THREADS / (TT * TU), NV = TV / VSTEP;
+        const uint32_t plane = (uint32_t)p.LT * (uint32_t)p.LU, nsite = plane * (uint32_t)p.LV;
+        const uint32_t lt = tid & (TT - 1), lu = (tid >> G::LT) & (TU - 1), lv0 = (uint32_t)tid >> (G::LT + G::LU);
+        int gt = gt0 + (int)lt, gu = gu0 + (int)lu, gv = gv0 + (int)lv0;
+        gt -= (gt >= p.LT) ? p.LT : 0;
+        gu -= (gu >= p.LU) ? p.LU : 0;
+        gv -= (gv >= p.LV) ? p.LV : 0;
+        uint32_t idx = (uint32_t)gt + (uint32_t)p.LT * (uint32_t)gu + plane * (uint32_t)gv;
+        T v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            v[k] = glat[idx];
+            idx += plane * VSTEP;
+            idx -= (idx >= nsite) ? nsite : 0u;
+        }
+        between();
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const uint32_t lv = lv0 + (uint32_t)(k * VSTEP);
+            tile[PAD ? G::pack(lt, lu, lv) : lt + TT * (lu + TU * lv)] = v[k];
+        }
+    } else {
+        tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
+        between();
+    }
+}
+
 template <class M, int TT, int TU, int TV, bool DENSE, int THREADS, int PAD, bool COUNT = false>
 __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat, typename M::site_t* tile,
                                           const UnitArgs& p, uint32_t tileId) {
@@ -135,21 +172,25 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
     static_assert(THREADS >= 64, "M::init fills its tables with the first 64 threads");
 
     const int tid = threadIdx.x;
-    const int kt = (int)(tileId % (uint32_t)p.ntile_t);
-    const int ku = (int)((tileId / (uint32_t)p.ntile_t) % (uint32_t)p.ntile_u);
-    const int kv = (int)(tileId / (uint32_t)(p.ntile_t * p.ntile_u));
+    const uint32_t q1 = udiv_small(tileId, (uint32_t)p.ntile_t);
+    const uint32_t q2 = udiv_small(q1, (uint32_t)p.ntile_u);
+    const int kt = (int)(tileId - q1 * (uint32_t)p.ntile_t);
+    const int ku = (int)(q1 - q2 * (uint32_t)p.ntile_u);
+    const int kv = (int)q2;
     const int gt0 = p.ot + kt * TT, gu0 = p.ou + ku * TU, gv0 = p.ov + kv * TV;
 
-    // the random words of the first group of four iterations, drawn beside
-    // the tile load (they do not depend on the lattice)
-    uint4 ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, 0u), p.seed, p.stream);
-    uint4 sw[SPT];
+    // HBM -> LDS (periodic wrap; origins are < one tile edge, so one
+    // subtraction suffices); while the loads are in flight, the random words
+    // of the first group of four iterations and the move set's LDS tables
+    uint4 ow, sw[SPT];
+    tile_load<TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid, [&] {
+        ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, 0u), p.seed, p.stream);
 #pragma unroll
-    for (int k = 0; k < SPT; k++)
-        sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, (uint32_t)(tid + k * THREADS), 0u), p.seed, p.stream);
-
-    // HBM -> LDS (periodic wrap; origins are < one tile edge, so one subtraction suffices)
-    tile_copy<true, TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid);
+        for (int k = 0; k < SPT; k++)
+            sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, (uint32_t)(tid + k * THREADS), 0u), p.seed,
+                                        p.stream);
+        M::template init<G>(tile, tid);
+    });
 
     // slot k of this thread: cell corner (+ j along t for the dense cosets).
     // Dense coset j of substep offset c sits at corner + (j, (c&3) + 3j,
@@ -165,7 +206,6 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
         ju[k] = (3u * j) << G::SU;
         vb[k] = base[k] | ((2u * (j & 1u)) << G::SV);
     }
-    M::template init<G>(tile, tid);
     __syncthreads();
     uint32_t nEmpty = 0, nFace = 0, nAcc = 0;   // COUNT: outcomes of this thread's attempts
 
