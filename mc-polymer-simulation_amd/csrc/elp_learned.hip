@@ -1052,8 +1052,8 @@ __global__ __launch_bounds__(256, 2) void lp_ring_time_unit2(uint8_t* __restrict
         const uint32_t offw = word_of(ow, it & 3);
         // the slot's two Philox blocks of this iteration: counter 2it in lane c
         // (g = 0, substeps 0-1), 2it + 1 in lane c + 16 (g = 1, substeps 2-3)
-        const uint4 rw = philox4x32_10(make_uint4(p.tau, tileId, (uint32_t)slot | 0x40000000u,
-                                                  2u * (uint32_t)it + (uint32_t)(g & 1)), p.seed, p.stream);
+        const uint4 rw = philox4x32_10_lanes(make_uint4(p.tau, tileId, (uint32_t)slot | 0x40000000u,
+                                                        2u * (uint32_t)it + (uint32_t)(g & 1)), p.seed, p.stream);
 #pragma unroll 1
         for (int s = 0; s < 4; s++) {
             const uint32_t c6 = (offw >> (8 * s)) & 0x3fu;
