@@ -181,14 +181,19 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
 
     // HBM -> LDS (periodic wrap; origins are < one tile edge, so one
     // subtraction suffices); while the loads are in flight, the random words
-    // of the first group of four iterations and the move set's LDS tables
+    // of the first group of four iterations (the empty statements keep the
+    // compiler from sinking them past the barrier to their first use) and the
+    // move set's LDS tables
     uint4 ow, sw[SPT];
     tile_load<TT, TU, TV, THREADS, PAD>(glat, tile, p, gt0, gu0, gv0, tid, [&] {
         ow = philox4x32_10(make_uint4(p.tau, tileId, kOffsetStream, 0u), p.seed, p.stream);
+        asm volatile("" ::"s"(ow.x), "s"(ow.y), "s"(ow.z), "s"(ow.w));
 #pragma unroll
-        for (int k = 0; k < SPT; k++)
+        for (int k = 0; k < SPT; k++) {
             sw[k] = philox4x32_10_lanes(make_uint4(p.tau, tileId, (uint32_t)(tid + k * THREADS), 0u), p.seed,
                                         p.stream);
+            asm volatile("" ::"v"(sw[k].x), "v"(sw[k].y), "v"(sw[k].z), "v"(sw[k].w));
+        }
         M::template init<G>(tile, tid);
     });
 
