@@ -124,12 +124,12 @@ int elp_set_threads(elp_engine* e, int threads);
  *     rate 1; the move set is not reversible, and its equilibrium depends on
  *     the schedule (cubic vs dense sub-lattice).
  *   ELP_RULES_BALANCED: the forced TransBack is accepted with probability 1/4
- *     (random bits 1-2 of its byte zero), and every slot of every substep
- *     draws its move from bits 6-7 of its byte (TransForw 1/2, DiffuseSL
- *     1/4, TransBack 1/4) instead of the fixed sequence.  Every move then
- *     has the probability of its reverse within one substep, each substep
- *     preserves the uniform measure over lattice states, and every schedule
- *     samples it.
+ *     (random bits 1-2 of its byte zero), and every substep of every tile
+ *     draws its move from bits 6-7 of the tile's site-offset byte
+ *     (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4) instead of the fixed
+ *     sequence.  Every move then has the probability of its reverse within
+ *     one substep, each substep preserves the uniform measure over lattice
+ *     states, and every schedule samples it.
  * The linear engine has the reference rates only.
  */
 enum { ELP_RULES_REFERENCE = 0, ELP_RULES_BALANCED = 1 };

@@ -283,8 +283,8 @@ __device__ __forceinline__ bool forw_move_first(uint32_t s, uint32_t ns, uint32_
 // its own: next = 0 puts the partner on the site itself, and neither end
 // then holds stored length.
 // Returns the outcome (kMoveEmpty ... kMoveAccepted) for counting launches.
-template <class G, bool PAIRS = false>
-__device__ __forceinline__ uint32_t trans_forw2(uint8_t* lat, uint32_t P, uint32_t r) {
+template <class G, bool PAIRS = false, class F = NoFill>
+__device__ __forceinline__ uint32_t trans_forw2(uint8_t* lat, uint32_t P, uint32_t r, const F& fill = F{}) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
@@ -304,6 +304,7 @@ __device__ __forceinline__ uint32_t trans_forw2(uint8_t* lat, uint32_t P, uint32
     const uint32_t na = G::addr(Pn), da = G::addr(Pd);
     uint32_t ns = lat[na];
     const uint32_t ds = lat[da];
+    fill();
     // the parts of both outcomes that need neither read, computed while the
     // reads are in flight (the empty statement keeps them ahead of the wait)
     const uint32_t s1mf = nb1 | ((s >> 1) & 0x10u), s1no = nb1 | (s & 0x70u), d1mf = nb2 | (s & 0x10u);
@@ -325,7 +326,7 @@ __device__ __forceinline__ uint32_t trans_forw2(uint8_t* lat, uint32_t P, uint32
 // rules = 1 (balanced): a fold forced by stored length at exactly one end
 // (reference rate 1) is accepted with probability 1/4 (r bits 1-2 zero), the
 // rate of the TransForw that undoes it, so every move and its reverse have
-// equal probabilities (with the per-slot move choice of RingMoves::move).
+// equal probabilities (with the move choice of RingMoves::move).
 // Bond next + srcNext as one bond: bits 0-3 the combined bond, bit 4 set when
 // it is a bond (bond_add_valid).  TABLE: read it from the 256-byte LDS table
 // (entry 16 next + srcNext).  When it is a bond its displacement is the sum
@@ -416,14 +417,15 @@ __device__ __forceinline__ uint32_t trans_back2(uint8_t* lat, uint32_t P, uint32
     return s == 0u ? kMoveEmpty : G::out(Ps | Pw) ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
 }
 
-template <class G>
-__device__ __forceinline__ uint32_t diffuse_sl2(uint8_t* lat, uint32_t P) {
+template <class G, class F = NoFill>
+__device__ __forceinline__ uint32_t diffuse_sl2(uint8_t* lat, uint32_t P, const F& fill = F{}) {
     const uint32_t a = G::addr(P);
     uint32_t s = lat[a];
     const uint32_t next = s & 0xfu;
     const uint32_t Pn = P + G::delta(next);
     const uint32_t na = G::addr(Pn);
     uint32_t ns = lat[na];
+    fill();
     // what the site byte fixes, computed while the read is in flight (the
     // empty statement keeps it ahead of the wait)
     const DiffuseParts dp = diffuse_parts(s);
@@ -439,63 +441,6 @@ __device__ __forceinline__ uint32_t diffuse_sl2(uint8_t* lat, uint32_t P) {
         lat[na] = (uint8_t)n1;
     }
     return s == 0u ? kMoveEmpty : !inTile ? kMoveFace : ok ? kMoveAccepted : kMoveRejected;
-}
-
-// Balanced rates with the LDS tables: the three moves a slot may draw
-// (r bits 6-7: 0,1 TransForw; 2 DiffuseSL; 3 TransBack) in one pass, so a
-// wave whose lanes drew different moves waits for three dependent LDS round
-// trips instead of the eight of the three functions in turn.  All three read
-// the site, then its bond partner (TransForw's n, DiffuseSL's n, TransBack's
-// source) beside the fold-pair word; then TransForw reads its target P + nb1
-// and TransBack its fold site Pn + srcNext, one read of the drawn one,
-// beside the combine code.  The outcome and the writes are those of
-// trans_forw2 / diffuse_sl2 / trans_back2 (rules = 1).
-template <class G, class F = NoFill>
-__device__ __forceinline__ uint32_t balanced_move(uint8_t* lat, uint32_t P, uint32_t r, const F& fill = F{}) {
-    const uint2* tab = ring_tab<G>(lat);
-    const uint32_t ty = r >> 6;
-    const bool isTF = ty < 2u, isDSL = ty == 2u, isTB = ty == 3u;
-    const uint32_t a = G::addr(P);
-    const uint32_t s = lat[a];
-    const uint32_t next = s & 0xfu;
-    const uint32_t Pn = P + G::delta(next);                 // bond partner (TransBack: its source)
-    const uint32_t na = G::addr(Pn);
-    const uint32_t wf = tab[(next << 2) | (r & 3u)].x;      // fold pair | offset of its first bond
-    const uint32_t ns = lat[na];
-    const uint32_t cc = ring_combine_tab<G>(lat)[(next << 4) | (ns & 0xfu)];
-    const uint32_t pair = wf >> 24;
-    // TransBack's fold site P + (next + srcNext) = Pn + delta(srcNext), read beside the combine code
-    const uint32_t X = isTB ? Pn + G::delta(ns & 0xfu) : P + wf;
-    const uint32_t xa = G::addr(X);
-    const uint32_t x = lat[xa];
-    fill();
-    const uint32_t label = s & 0x30u, sl = s & 0x40u, nsl = ns & 0x40u, xsl = x & 0x40u;
-    const uint32_t nb1 = pair & 0xfu, nb2 = pair >> 4, newNext = cc & 0xfu;
-    // TransForw: an empty target next to both ends, stored length at one of them
-    const bool okTF = isTF & !G::out(Pn | X) & ((sl | nsl) != 0u) & (x == 0u);
-    const bool mfTF = forw_move_first(s, ns, r);
-    const uint32_t sTF = mfTF ? (nb1 | ((label >> 1) & 0x10u)) : (nb1 | label | sl);
-    const uint32_t xTF = nb2 | (mfTF ? (label & 0x10u) : ((ns & 0x20u) >> 1));
-    // DiffuseSL: stored length at exactly one end
-    const bool okDSL = isDSL & !G::out(Pn) & (((ns ^ s) & 0x40u) != 0u);
-    const DiffuseParts dp = diffuse_parts(s);
-    const uint32_t sDSL = diffuse_site(dp, ns);
-    const uint32_t nDSL = diffuse_partner(dp, ns);
-    // TransBack: fold the source onto the site or the third corner; a fold
-    // forced by stored length at one end is taken with probability 1/4
-    const bool okTB = isTB & (s != 0u) & !G::out(Pn | X) & back_ok_bits(s, ns, x, cc) &
-                      !(((sl | xsl) != 0u) & ((r & 6u) != 0u));
-    const uint32_t mfTB = back_move_first(s, x, r);
-    const BackParts bp = back_parts(s, ns);
-    const uint32_t sTB = back_site(mfTB, bp, newNext);
-    const uint32_t xTB = back_target(mfTB, bp, x);
-    if (okTF | okDSL | okTB) {
-        lat[a] = (uint8_t)(okTF ? sTF : okDSL ? sDSL : sTB);
-        if (okDSL | okTB | (okTF & !mfTF)) lat[na] = (uint8_t)(okTF ? (ns & 0x1fu) : okDSL ? nDSL : 0u);
-        if (okTF | okTB) lat[xa] = (uint8_t)(okTF ? xTF : xTB);
-    }
-    const bool face = isDSL ? G::out(Pn) : G::out(Pn | X);
-    return s == 0u ? kMoveEmpty : face ? kMoveFace : (okTF | okDSL | okTB) ? kMoveAccepted : kMoveRejected;
 }
 
 // ringpol3's substep sequence {TransForw, DiffuseSL, TransForw, TransBack}
@@ -522,24 +467,20 @@ struct RingMovesT {
     __device__ static __forceinline__ uint32_t move(uint8_t* lat, uint32_t P, int s, uint32_t r, uint32_t rules,
                                                     const F& fill = F{}) {
         if (rules) {
-            // balanced rates: every slot draws its move from bits 6-7 of its
-            // byte (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the
-            // proportions of the reference sequence), so each substep is a
-            // symmetric kernel and preserves the uniform measure by itself
-            if constexpr (PAIRS) {
-                return balanced_move<G>(lat, P, r, fill);
-            } else {
-                fill();
-                const uint32_t ty = r >> 6;
-                if (ty < 2u) return trans_forw2<G, PAIRS>(lat, P, r);
-                if (ty == 2u) return diffuse_sl2<G>(lat, P);
-                return trans_back2<G, PAIRS>(lat, P, r, rules);
-            }
+            // balanced rates: the move is drawn by bits 6-7 of the byte
+            // (TransForw 1/2, DiffuseSL 1/4, TransBack 1/4 -- the proportions
+            // of the reference sequence), so each substep is a symmetric
+            // kernel and preserves the uniform measure by itself.  The tile
+            // loop gives every slot of a substep the same two bits (from the
+            // tile-uniform offset byte), so this is one scalar branch.
+            const uint32_t ty = __builtin_amdgcn_readfirstlane(r >> 6);
+            if (ty < 2u) return trans_forw2<G, PAIRS>(lat, P, r, fill);
+            if (ty == 2u) return diffuse_sl2<G>(lat, P, fill);
+            return trans_back2<G, PAIRS>(lat, P, r, rules, fill);
         }
-        if (s == 1) { fill(); return diffuse_sl2<G>(lat, P); }
+        if (s == 1) return diffuse_sl2<G>(lat, P, fill);
         if (s == 3) return trans_back2<G, PAIRS>(lat, P, r, rules, fill);
-        fill();
-        return trans_forw2<G, PAIRS>(lat, P, r);
+        return trans_forw2<G, PAIRS>(lat, P, r, fill);
     }
 };
 using RingMoves = RingMovesT<false>;

@@ -17,7 +17,10 @@
 // Randomness (Philox4x32-10, key = (seed, stream)):
 //   site offsets : counter (tau, tile, 0xFFFFFFFF, iter/4), word iter%4,
 //                  byte s = coset of substep s (tile-uniform);
-//   moves        : counter (tau, tile, slot, iter/4), word iter%4, byte s.
+//   moves        : counter (tau, tile, slot, iter/4), word iter%4, byte s;
+//                  under balanced rates its bits 6-7 (the move drawn) are
+//                  replaced by bits 6-7 of the site-offset byte, so every
+//                  slot of a substep makes the same kind of move.
 // The tile index is the replica-local logical tile, so a replica gives the
 // same result alone or inside a batch, whatever the launch geometry.
 #pragma once
@@ -263,7 +266,10 @@ __device__ __forceinline__ void tile_unit(typename M::site_t* __restrict__ glat,
                         uint32_t P;
                         if (DENSE) P = ((cu + ju[k]) & (3u << G::SU)) | (vb[k] ^ cv);
                         else P = base[k] + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                        const uint32_t rk = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
+                        // balanced rates: bits 6-7 (the move drawn) from the
+                        // tile-uniform offset byte, the same for every slot
+                        const uint32_t rb = (word_of(sw[k], j) >> (8 * s)) & 0xffu;
+                        const uint32_t rk = RULES ? ((rb & 0x3fu) | ((offw >> (8 * s)) & 0xc0u)) : rb;
                         // the last slot of the first iteration's last substep draws
                         const uint32_t out = (M::DRAW_AHEAD && j == 0 && s == M::NSUB - 1 && k == SPT - 1)
                                                  ? M::template move<G>(tile, P, s, rk, RULES, draw)
@@ -358,7 +364,9 @@ __device__ __forceinline__ void tile_unit_compact(typename M::site_t* __restrict
                 const uint32_t cu = (c & 3u) << G::SU, cv = (c >> 2) << G::SV;
                 const uint32_t P = DENSE ? (((cu + ju) & (3u << G::SU)) | (vb ^ cv))
                                          : base + G::pack(c & 3u, (c >> 2) & 3u, (c >> 4) & 3u);
-                const uint32_t r = (word_of(sw, jj) >> (8 * s)) & 0xffu;
+                const uint32_t rb = (word_of(sw, jj) >> (8 * s)) & 0xffu;
+                // balanced rates: bits 6-7 from the tile-uniform offset byte (tile_unit)
+                const uint32_t r = p.rules ? ((rb & 0x3fu) | ((offw >> (8 * s)) & 0xc0u)) : rb;
                 const bool occ = tile[G::addr(P)] != 0;
                 const uint64_t bal = __ballot(occ);
                 if (lane == 0) cnt[w] = (uint32_t)__popcll(bal);

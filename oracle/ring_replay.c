@@ -299,7 +299,9 @@ int orc_ring_run_rules(uint8_t* lat, int LT, int LU, int LV, int TT, int TU, int
                                 slot_site(sublat, TT, TU, slot, c, &t, &u, &v);
                                 Draw rd = {(philox_word(tau, tile, (uint32_t)slot, (uint32_t)it, seed, stream) >> (8 * s)) & 0xff};
                                 /* move of this slot: the substep's (TF, DSL, TF, TB), or under
-                                   balanced rates bits 6-7 of the byte: 0,1 TF, 2 DSL, 3 TB */
+                                   balanced rates bits 6-7 of the byte: 0,1 TF, 2 DSL, 3 TB, those two
+                                   bits taken from the tile's offset byte (the same for every slot) */
+                                if (rules) rd.fixed = (rd.fixed & 0x3fu) | ((ow >> (8 * s)) & 0xc0u);
                                 int mv = s == 1 ? 1 : (s == 3 ? 2 : 0);
                                 if (rules) mv = (rd.fixed >> 6) < 2 ? 0 : (int)(rd.fixed >> 6) - 1;
                                 const int empty = X.b[t + TT * (u + TU * v)] == 0;
