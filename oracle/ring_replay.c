@@ -16,7 +16,9 @@
  * unit i the tile grid origin is origins[3i..3i+2]; Philox4x32-10 with key
  * (seed, stream) gives the tile-uniform coset per substep (counter
  * (tau, tile, 0xFFFFFFFF, iter/4), word iter%4, byte substep) and the move
- * bits per slot (counter (tau, tile, slot, iter/4), same word/byte).
+ * bits per slot (counter (tau, tile, slot, iter/4), same word/byte); under
+ * balanced rates bits 6-7 of the slot's byte (the move drawn) are those of
+ * the tile's coset byte.
  *
  * Pinned by: Philox known-answer vectors (Random123 kat_vectors); slot-order
  * independence (ascending, descending and shuffled orders give identical

@@ -31,8 +31,9 @@ the chain is not reversible and its equilibrium is not the uniform measure
 over its states (stored length per site <= 2) that efpol's table targets.
 
 With balanced rates (ELP_RULES_BALANCED: the forced TransBack accepted with
-probability 1/4, and each slot drawing TransForw / DiffuseSL / TransBack with
-probabilities 1/2, 1/4, 1/4 -- the proportions used as weights here) every
+probability 1/4, and each substep of each tile drawing TransForw / DiffuseSL /
+TransBack with probabilities 1/2, 1/4, 1/4 -- the proportions used as
+weights here, which are each slot's marginal probabilities) every
 ratio must be exactly 1: each substep is then a symmetric kernel, so the
 uniform measure is stationary for every schedule.
 """
