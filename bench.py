@@ -9,7 +9,9 @@ the dense ring melt the target is quoted on: L=128, N=1000, density 1.2
 stored-length reshuffle) from seed 0 and relaxed by the warm-up steps.
 
 Prints one JSON line.  value = move attempts/s (every slot of every substep
-counts as one attempt, as in the reference kernel).  roofline.achieved =
+counts as one attempt, as in the reference kernel), by the wall clock around
+exactly `--steps` time units with a synchronize of the engine's stream on
+both sides (the only stream with work: the process does not load torch).  roofline.achieved =
 algorithmic HBM bytes per launch (the tile is read and written once:
 2 B/site x L^3) / average launch time from HIP events on the engine's stream.
 cpu_baseline = the reference CPU engine efpol (cpupol/, built from its own
