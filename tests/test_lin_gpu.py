@@ -37,6 +37,24 @@ def test_lin_bit_exact_vs_replay(tile, sublat, n_iter, N, seed):
     assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} sites differ"
 
 
+def test_lin_odd_tile_counts_bit_exact():
+    """An 80 x 112 x 48 box: 5, 7 and 3 tiles of 16 per axis (the tile
+    coordinates come from a float reciprocal with one correction step)."""
+    print("[lin parity] box 80x112x48 dense 16^3", flush=True)
+    h = elp.LinHost(N=200, L=(80, 112, 48), density=1.2, seed=7, box_unit=16)
+    assert h.L == (80, 112, 48)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    org = h.origins(tile, 6)
+    eng = elp.Engine("linear", h.L, tile, elp.SUBLAT_DENSE, 16)
+    eng.upload(lat0)
+    eng.run(org, tau0=0, seed=h.seed)
+    want = replay_lin(lat0, h.L, tile, elp.SUBLAT_DENSE, 16, org, 0, h.seed)
+    assert np.count_nonzero(want != lat0) > 1000
+    assert np.array_equal(eng.download(), want)
+    eng.close()
+
+
 def test_lin_full_size_invariants():
     """Config of the linear target: L=128, N=500 (5033 chains), 100 time units."""
     print("[lin full size] L=128 N=500", flush=True)

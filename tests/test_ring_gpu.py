@@ -184,6 +184,29 @@ def test_ring_non_cubic_box_bit_exact():
     eng.close()
 
 
+@pytest.mark.parametrize("sublat,n_iter,rules", [(elp.SUBLAT_DENSE, 13, elp.RULES_REFERENCE),
+                                                  (elp.SUBLAT_CUBIC, 51, elp.RULES_BALANCED)])
+def test_ring_odd_tile_counts_bit_exact(sublat, n_iter, rules):
+    """An 80 x 112 x 48 box: 5, 7 and 3 tiles of 16 per axis.  The kernels
+    derive a tile's coordinates from its index with a float reciprocal and
+    one correction step; tile counts that are not powers of two exercise it."""
+    _say(f"[ring parity] box 80x112x48 sublattice {sublat} rules {rules}")
+    h = elp.RingHost(N=200, L=(80, 112, 48), density=1.2, seed=6, box_unit=16)
+    assert h.L == (80, 112, 48)
+    lat0 = h.lattice()
+    tile = (16, 16, 16)
+    h.load_shift(tile)
+    org = h.origins(tile, 6)
+    eng = elp.Engine("ring", h.L, tile, sublat, n_iter)
+    eng.set_rules(rules)
+    eng.upload(lat0)
+    eng.run(org, tau0=5, seed=h.seed)
+    want = replay_ring(lat0, h.L, tile, sublat, n_iter, org, 5, h.seed, rules=rules)
+    assert np.count_nonzero(want != lat0) > 1000
+    assert np.array_equal(eng.download(), want)
+    eng.close()
+
+
 _COMPACT_CHECK = r"""
 import sys
 import numpy as np
