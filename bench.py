@@ -11,9 +11,11 @@ stored-length reshuffle) from seed 0 and relaxed by the warm-up steps.
 Prints one JSON line.  value = move attempts/s (every slot of every substep
 counts as one attempt, as in the reference kernel), by the wall clock around
 exactly `--steps` time units with a synchronize of the engine's stream on
-both sides (the only stream with work: the process does not load torch).  roofline.achieved =
-algorithmic HBM bytes per launch (the tile is read and written once:
-2 B/site x L^3) / average launch time from HIP events on the engine's stream.
+both sides (the only stream with work: the process does not load torch).
+roofline.achieved = algorithmic HBM bytes per launch (the tile is read and
+written once: 2 B/site x L^3) / average launch time from HIP events on the
+engine's stream.  --rules balanced times the ring engine with balanced rates
+(the benchmark line keeps the reference's).
 cpu_baseline = the reference CPU engine efpol (cpupol/, built from its own
 sources into oracle/_ref) timed on the host cores (the process's CPU affinity
 mask), one independent process per core, on a bounded sample; the byte-exact
@@ -419,6 +421,8 @@ STATICS = {
              "stored-length ratio about 1 % above the reference schedule's (the move set is not reversible)",
     "cubic": "reference rates on the reference's cubic sub-lattice (one site per 4^3 cell): the reference's statics",
 }
+STATICS_BALANCED = ("balanced rates (forced TransBack at 1/4, the move kind drawn per tile and substep): the "
+                    "uniform measure over lattice states is stationary, the same statics on every schedule")
 
 
 def binding_resource(kernel, chain, L, N, tile, sublattice, avg_launch_us):
@@ -444,7 +448,7 @@ def binding_resource(kernel, chain, L, N, tile, sublattice, avg_launch_us):
     return None
 
 
-def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, warmup):
+def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, warmup, rules=0):
     """Time `steps` time units of one schedule from lat0 (after `warmup` untimed ones), then replay the same
     time units with move counting from the same lattice (the counting kernels move identically, which is
     checked) for the accepted-move rate and the rejection shares."""
@@ -452,6 +456,8 @@ def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, war
     n_iter = n_iter if n_iter is not None else ((13 if dense else 51) if ring else (16 if dense else 64))
     eng = elp.Engine("ring" if ring else "linear", host.L, tile, elp.SUBLAT_DENSE if dense else elp.SUBLAT_CUBIC,
                      n_iter)
+    if rules:
+        eng.set_rules(rules)
     eng.upload(lat0)
     host.set_lattice(lat0)
     host.load_shift(tile)
@@ -495,7 +501,7 @@ def measure_schedule(elp, host, lat0, ring, tile, sublattice, n_iter, steps, war
     a = max(c["attempts"], 1)   # a run of zero iterations (fixed-cost measurement) attempts nothing
     launch_s = kernel_ms * 1e-3 / steps
     return {
-        "tile": list(tile), "sublattice": sublattice, "n_iter": n_iter, "threads": info["threads"],
+        "tile": list(tile), "sublattice": sublattice, "n_iter": n_iter, "rules": rules, "threads": info["threads"],
         "blocks": info["blocks"], "attempts_per_step": attempts_per_step, "wall_s": wall,
         "attempts_per_s": attempts_per_step * steps / wall,
         "accepted_moves_per_s": c["accepted"] / wall,
@@ -512,7 +518,7 @@ def schedule_entry(m, headline):
     return {
         "headline": headline,
         "tile": m["tile"], "sublattice": m["sublattice"], "iterations_per_time_unit": m["n_iter"],
-        "statics": STATICS[m["sublattice"]],
+        "statics": STATICS_BALANCED if m["rules"] else STATICS[m["sublattice"]],
         "attempts_per_s": m["attempts_per_s"], "accepted_moves_per_s": m["accepted_moves_per_s"],
         "ms_per_step": m["ms_per_step"], "avg_launch_us": m["avg_launch_us"],
         "roofline_frac": m["achieved_GBs"] / HBM_PEAK_GBS, "move_outcomes": m["move_outcomes"],
@@ -527,6 +533,8 @@ def main():
     ap.add_argument("--L", type=int, default=128)
     ap.add_argument("--N", type=int, default=None, help="monomers per chain (ring 1000, linear 500)")
     ap.add_argument("--chain", default="ring", choices=["ring", "linear"])
+    ap.add_argument("--rules", default="reference", choices=["reference", "balanced"],
+                    help="ring move rates (the benchmark line: the reference's)")
     ap.add_argument("--density", type=float, default=1.2)
     ap.add_argument("--tile", default="16")
     ap.add_argument("--sublattice", default="dense", choices=["cubic", "dense"])
@@ -578,8 +586,13 @@ def main():
     # Measured first: the first timed run of a process pays a one-time runtime cost (about 20 us at the
     # first burst of launches), which then falls on the companion, not on the headline schedule.
     other_sub = "cubic" if args.sublattice == "dense" else "dense"
-    other = measure_schedule(elp, host, lat0, ring, (16, 16, 16), other_sub, None, args.steps, args.warmup)
-    head = measure_schedule(elp, host, lat0, ring, tile, args.sublattice, args.n_iter, args.steps, args.warmup)
+    rules = elp.RULES_BALANCED if args.rules == "balanced" else elp.RULES_REFERENCE
+    if rules and not ring:
+        print("bench.py: --rules balanced applies to the ring engine only", file=sys.stderr)
+        return 2
+    other = measure_schedule(elp, host, lat0, ring, (16, 16, 16), other_sub, None, args.steps, args.warmup, rules)
+    head = measure_schedule(elp, host, lat0, ring, tile, args.sublattice, args.n_iter, args.steps, args.warmup,
+                            rules)
 
     nsite = host.L[0] * host.L[1] * host.L[2]
     site_bytes = 1 if ring else 2
@@ -620,6 +633,7 @@ def main():
                         f"({host.n_pol} chains), one time unit per step",
             "tile": list(tile),
             "sublattice": args.sublattice,
+            "rules": args.rules,
             "iterations_per_time_unit": head["n_iter"],
             "attempts_per_site_per_time_unit": head["attempts_per_step"] / nsite,
             "threads_per_block": head["threads"],
@@ -640,8 +654,9 @@ def main():
             "avg_launch_us": head["avg_launch_us"],
             "copy_kernel_same_bytes_GBs": copy_gbs,
         },
-        "binding": binding_resource(kernel, args.chain, host.L[0], host.N, tile, args.sublattice,
-                                    head["avg_launch_us"]),
+        # the recorded SQ counters are those of the reference rates
+        "binding": None if rules else binding_resource(kernel, args.chain, host.L[0], host.N, tile,
+                                                       args.sublattice, head["avg_launch_us"]),
         "schedules": [schedule_entry(head, True), schedule_entry(other, False)],
         "cpu_baseline": cpu,
     }
