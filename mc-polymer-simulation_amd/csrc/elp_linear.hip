@@ -384,10 +384,15 @@ __device__ __forceinline__ uint32_t lin_trans_move_interior(uint16_t* lat, uint3
     const uint32_t bondS = pbS ? prev : next;
     const uint32_t pair = wp >> 24;
     const uint32_t nPrev = pair & 0xfu, nNext = pair >> 4;
-    // split: X2 = bond partner, X1 = inserted site; fold: X2 = Pn, X1 = Pp
+    // split: X2 = bond partner, X1 = inserted site; fold: X2 = Pn, X1 = Pp.
+    // X1 as a mask blend of its two candidates (a ternary over them compiles
+    // to a divergent branch whose split arm waits for the table word before
+    // the fold arm may start)
     const uint32_t c2 = sl ? (pbS ? (bondS ^ 0xfu) : bondS) : next;
     const uint32_t X2 = P + G::delta(c2);
-    const uint32_t X1 = sl ? (pbS ? X2 : P) + wp : P + G::delta(prev ^ 0xfu);
+    const uint32_t X1split = (pbS ? X2 : P) + wp, X1fold = P + G::delta(prev ^ 0xfu);
+    const uint32_t slm = 0u - (uint32_t)sl;
+    const uint32_t X1 = (X1split & slm) | (X1fold & ~slm);
     const uint32_t a1 = G::addr(X1), a2 = G::addr(X2);
     const uint32_t x1 = lat[a1];
     const uint32_t x2 = lat[a2];
